@@ -1,0 +1,244 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X bubble finder (one JSON line on stdout).
+
+Workload (the metric's configuration): the batched exact finder at 512^3
+float32, side 400 Mpc, 10 redshift slices, radii 100 Mpc * 0.9^k down to the
+cell size (47 radii), 2.5e5 point sources per slice.  Slice s uses the same
+Gaussian field g rescaled by its linear growth factor, g_s = (D_s/0.25) g with
+D_s = linspace(0.10, 0.25, 10), 1+delta_s = exp(g_s)/<exp(g_s)>, sources drawn
+with P ~ (1+delta_s)^2 and lognormal strengths, rec_s = (1+delta_s)^2.
+
+A step is one pass of all radii over all slices (inputs resident in HBM).
+`value` = slices * N^3 * n_radii / step time.  The plain per-radius GPU path
+(the reference's dataflow) is timed on one slice and scaled to ten slices.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "voxel·radius updates/s at 512^3 (batched vs plain GPU); grad/forward time"
+HBM_PEAK_GBS = 8000.0
+
+
+def make_inputs(torch, n, side, n_slices, n_src, seed=3001, slope=-2.5, sigma=1.2):
+    """Synthetic slices on the device (float32 boxes, reference layout)."""
+    dev = "cuda"
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    white = torch.randn((n, n, n), generator=gen, device=dev, dtype=torch.float32)
+    spec = torch.fft.rfftn(white)
+    del white
+    kx = torch.fft.fftfreq(n, device=dev) * n
+    kz = torch.arange(n // 2 + 1, device=dev, dtype=torch.float32)
+    k2 = kx[:, None, None] ** 2 + kx[None, :, None] ** 2 + kz[None, None, :] ** 2
+    amp = torch.where(k2 > 0, k2.clamp_min(1e-30) ** (slope / 4.0), torch.zeros_like(k2))
+    spec *= amp
+    del amp, k2
+    g = torch.fft.irfftn(spec, s=(n, n, n))
+    del spec
+    g = (g - g.mean()) / g.std() * sigma
+    growth = np.linspace(0.10, 0.25, n_slices)
+    recs, ions, means = [], [], []
+    for s in range(n_slices):
+        gs = g * float(growth[s] / 0.25)
+        e = torch.exp(gs.double())
+        e /= e.mean()
+        delta1 = e  # 1 + delta
+        w = (delta1 ** 2).reshape(-1)
+        cdf = torch.cumsum(w, 0)
+        cdf /= cdf[-1].clone()
+        u = torch.rand(n_src, generator=gen, device=dev, dtype=torch.float64)
+        cells = torch.searchsorted(cdf, u).clamp_max(n ** 3 - 1)
+        strength = torch.exp(torch.randn(n_src, generator=gen, device=dev, dtype=torch.float64))
+        emis = torch.zeros(n ** 3, device=dev, dtype=torch.float64)
+        emis.index_add_(0, cells, strength)
+        recs.append((delta1 ** 2).float().reshape(-1).contiguous())
+        ions.append(emis.float().contiguous())
+        del e, w, cdf, emis, delta1
+    torch.cuda.synchronize()
+    return recs, ions
+
+
+def cpu_baseline(n, side, radii_full, seconds_hint=20.0):
+    """Oracle (CPU restatement, reference dataflow, FP32 typing) on a bounded
+    sample of the workload: one 512^3 slice with the first radii of the ladder."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_bind as orc
+    from simfast21_amd import synth
+    delta, emis, rec = synth.direct_boxes(n, side, 250000, 3001, growth=0.175 / 0.25)
+    nr = 2
+    t0 = time.perf_counter()
+    orc.hii_slice(n, side, radii_full[:nr], rec, emis, precision=32, dilation=0)
+    dt = time.perf_counter() - t0
+    cores = os.cpu_count() or 1
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except Exception:
+        pass
+    omp = os.environ.get("OMP_NUM_THREADS")
+    if omp:
+        cores = min(cores, int(omp))
+    return {
+        "value": n ** 3 * nr / dt,
+        "unit": "voxel·radius updates/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": (f"one {n}^3 slice, first {nr} radii of the ladder (reference dataflow: 2 r2c + "
+                   f"per radius filter, 2 c2r, criterion, ball + 2 r2c + c2r), FP32 typing, "
+                   f"own OpenMP CPU FFT (not FFTW); {dt:.1f} s wall incl. forward transforms"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--slices", type=int, default=10)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--radius-batch", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: the bubble finder's slices are independent and a "
+              f"multi-GPU run is not part of this build (asked for --gpus {args.gpus})",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import simfast21_amd as sf
+    from simfast21_amd import synth
+
+    n = args.n
+    side = 400.0 * n / 512
+    dx = side / n
+    radii = synth.geometric_ladder(100.0 * n / 512, 0.9, dx)
+    n_src = int(round(2.5e5 * (n / 512) ** 3))
+    S = args.slices
+    print(f"[bench] generating {S} slices at {n}^3 ({len(radii)} radii)", file=sys.stderr, flush=True)
+    recs, ions = make_inputs(torch, n, side, S, n_src)
+    outs = [torch.empty_like(r) for r in recs]
+
+    ctx = sf.HiiContext(n, side, radii, sf.INPUT_DIRECT, sf.PATH_FUSED,
+                        radius_batch=args.radius_batch)
+    torch.cuda.synchronize()
+    for _ in range(args.warmup):
+        _, neu = ctx.run_batch(recs, ions, out_list=outs)
+    torch.cuda.synchronize()
+    print("[bench] warmup done", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        _, neu = ctx.run_batch(recs, ions, out_list=outs)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms = (t1 - t0) * 1e3 / args.steps
+    units = S * n ** 3 * len(radii)
+    value = units / (ms * 1e-3)
+    print(f"[bench] fused {ms:.1f} ms/step, mean x_HII {[round(1 - v, 4) for v in neu]}",
+          file=sys.stderr, flush=True)
+
+    # per-kernel HIP-event timings on the context stream (profiled steps)
+    ctx.reset_stats()
+    ctx.profile(True)
+    prof_steps = max(1, min(2, args.steps))
+    tp0 = time.perf_counter()
+    for _ in range(prof_steps):
+        ctx.run_batch(recs, ions, out_list=outs)
+    torch.cuda.synchronize()
+    prof_ms = (time.perf_counter() - tp0) * 1e3 / prof_steps
+    stats = ctx.kernel_stats()
+    ctx.profile(False)
+    K = ctx_k = min(8 if args.radius_batch <= 0 else args.radius_batch, len(radii))
+    # algorithmic HBM bytes per launch of each fused kernel (complex64 = 8 B/voxel)
+    v = n ** 3
+    nb = math.ceil(len(radii) / K)
+    avg_k = len(radii) / nb
+    algo = {
+        "filter_i": v * 8 * (1 + avg_k),
+        "cols_inv_j": v * 8 * 2 * avg_k,
+        "inv_rows_crit": v * 8 * avg_k,
+        "fwd_rows": v * (8 + 8),
+        "cols_fwd_j": v * 16,
+    }
+    dom = max(stats.items(), key=lambda kv: kv[1][1])[0]
+    cnt, tot = stats[dom]
+    avg_ms = tot / cnt
+    roof = None
+    if dom in algo:
+        ach = algo[dom] / (avg_ms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                "avg_launch_ms": round(avg_ms, 4)}
+    # whole-step algorithmic roofline (per voxel-radius 80h+4 B, per voxel-slice 80h+25 B)
+    hfac = (n // 2 + 1) / n
+    step_bytes = S * v * (len(radii) * (80 * hfac + 4) + (80 * hfac + 25))
+    step_roof = {"bytes": step_bytes, "achieved_GBs": round(step_bytes / (ms * 1e-3) / 1e9, 1),
+                 "frac": round(step_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+
+    plain = None
+    if not args.no_plain:
+        del ctx
+        torch.cuda.empty_cache()
+        pctx = sf.HiiContext(n, side, radii, sf.INPUT_DIRECT, sf.PATH_PLAIN)
+        pout = torch.empty_like(recs[0])
+        pctx.run_slice(recs[S // 2], ions[S // 2], xhii=pout)
+        torch.cuda.synchronize()
+        tq0 = time.perf_counter()
+        _, pneu = pctx.run_slice(recs[S // 2], ions[S // 2], xhii=pout)
+        torch.cuda.synchronize()
+        pms = (time.perf_counter() - tq0) * 1e3
+        plain = {"ms_per_slice": round(pms, 2), "ms_per_step_10_slices_scaled": round(pms * S, 1),
+                 "speedup_fused_vs_plain": round(pms * S / ms, 2),
+                 "mean_xhii_slice": round(1 - pneu, 6), "fused_mean_xhii_slice": round(1 - neu[S // 2], 6)}
+        print(f"[bench] plain {pms:.1f} ms/slice", file=sys.stderr, flush=True)
+        del pctx
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(n, side, radii)
+        except Exception as e:  # the baseline is reported, never required
+            cpu = {"error": repr(e)}
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "voxel·radius updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"batched exact bubble finder, {n}^3 float32, side {side:g} Mpc, {S} slices, "
+                         f"{len(radii)} radii ({100.0 * n / 512:g} Mpc x 0.9^k to the cell size), "
+                         f"{n_src} sources/slice"),
+            "n": n, "slices": S, "radii": len(radii), "radius_batch": K,
+        },
+        "roofline": roof,
+        "step_roofline": step_roof,
+        "profiled_ms_per_step": round(prof_ms, 3),
+        "kernels_ms": {k: [c, round(t, 3)] for k, (c, t) in stats.items()},
+        "plain": plain,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,154 @@
+/* sf21_hii.h -- C-ABI of the MI355X (gfx950) ionization-field step of
+ * SimFast21: the excursion-set HII bubble finder of get_HIIbubbles.c.
+ *
+ * The reference has no library interface: its boundary is the program
+ * `get_HIIbubbles.x base_dir` (get_HIIbubbles.c:40-68) plus the files under
+ * base_dir, and inside it the FFTW-threaded loop get_HIIbubbles.c:98-488.
+ * This header is the thin C boundary that replaces that loop.  It is called
+ * by the drop-in CLI (simfast21_amd/host/get_HIIbubbles.c), by the Python
+ * package through ctypes, and by the benchmark.
+ *
+ * Conventions
+ *   - Every function returns int: SF21_OK (0) or a negative SF21_ERR_*; the
+ *     library never calls exit().  sf21_strerror() names a code.
+ *   - Boxes are float32, n^3, index i*n*n + j*n + p (get_HIIbubbles.c:338,
+ *     output_files.txt:21-25).  "_dev" entry points take device pointers
+ *     (hipMalloc'd memory on the context's device); the others host pointers.
+ *   - Lengths: the box side and the radii share one unit (the reference
+ *     works in Mpc/h: Input_variables.c:121-129).
+ *   - A context is single-threaded and bound to one HIP stream; separate
+ *     contexts may run concurrently.  The caller owns its buffers; the
+ *     context owns device scratch, tables and captured graphs.
+ */
+#ifndef SF21_HII_H
+#define SF21_HII_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SF21_OK 0
+#define SF21_ERR_ARG (-1)         /* invalid argument or unsupported size */
+#define SF21_ERR_ALLOC (-2)       /* device or host allocation failed */
+#define SF21_ERR_HIP (-3)         /* HIP runtime error (no device, launch failure) */
+#define SF21_ERR_IO (-4)          /* file missing, unreadable or short */
+#define SF21_ERR_FORMAT (-5)      /* malformed simfast21.ini line */
+
+/* Input modes of a slice. */
+#define SF21_INPUT_SFR 0     /* a = delta (deltanl box), b = SFRD box: the reference's
+                                use_SFR = T path, get_HIIbubbles.c:216-225, :262-275 */
+#define SF21_INPUT_DIRECT 1  /* a = recombination box, b = emissivity box, c = residual
+                                neutral fraction box (nullable -> 0) */
+
+/* Execution paths. */
+#define SF21_PATH_FUSED 0    /* fused exact finder: filtered fields of several radii per
+                                pass, exact ball dilation of all centres at once */
+#define SF21_PATH_PLAIN 1    /* the reference's dataflow on the GPU, one radius at a time:
+                                two real inverse transforms per radius, ball FFT
+                                convolution (main loop) and stencil (small loop), the
+                                any-centre flag read back each radius */
+
+typedef struct sf21_hii_ctx sf21_hii_ctx;
+
+typedef struct sf21_hii_config {
+  int n;                 /* N_smoothed; supported: 64, 96, 128, 192, 256, 512 (32 fused only) */
+  double box_len;        /* box side L (unit of the radii) */
+  int n_radii;           /* ladder length */
+  const double *radii;   /* descending ladder (get_HIIbubbles.c:298-461); radii >= 2*dx are
+                            the main loop, [dx, 2*dx) the small loop */
+  int input_mode;        /* SF21_INPUT_* */
+  int path;              /* SF21_PATH_* */
+  double hubble;         /* h, SFR mode (Rrec's cell volume, user_functions.c:62) */
+  double fesc;           /* escape fraction, SFR mode (get_HIIbubbles.c:273) */
+  int radius_batch;      /* radii per fused filter pass, 0 = automatic */
+  int device;            /* HIP device ordinal */
+  int filter_tile;       /* columns per block of the fused filter pass: 4, 8, 16 or 0 (auto) */
+} sf21_hii_config;
+
+const char *sf21_strerror(int code);
+
+/* Context lifetime: replaces the allocations and FFTW plans of
+ * get_HIIbubbles.c:98-191. */
+int sf21_hii_create(const sf21_hii_config *cfg, sf21_hii_ctx **out);
+int sf21_hii_destroy(sf21_hii_ctx *ctx);
+/* Use an external hipStream_t (e.g. torch's) instead of the context's own. */
+int sf21_hii_set_stream(sf21_hii_ctx *ctx, void *hip_stream);
+
+/* One redshift slice: replaces get_HIIbubbles.c:216-479 (everything between
+ * reading the input boxes and writing the xHII box).  z is used in SFR mode
+ * only.  *neutral receives mean(1 - xHII) reduced in FP64 (:473-477). */
+int sf21_hii_run_slice(sf21_hii_ctx *ctx, const float *a, const float *b, const float *c,
+                       double z, float *xhii, double *neutral);
+int sf21_hii_run_slice_dev(sf21_hii_ctx *ctx, const float *a, const float *b,
+                           const float *c, double z, float *xhii, double *neutral);
+
+/* Several slices of the same box size in one submission (device pointers);
+ * the whole sequence is captured once into a hipGraph and replayed.
+ * neutral: host array of n_slices doubles. */
+int sf21_hii_run_batch_dev(sf21_hii_ctx *ctx, int n_slices, const float *const *a,
+                           const float *const *b, const float *const *c, const double *z,
+                           float *const *xhii, double *neutral);
+
+/* Inspection for tests (fused path): per-cell ball bound d2max(R) of the
+ * largest radius at which the cell was a bubble centre in the last slice, -1
+ * if never (host output, n^3 ints). */
+int sf21_hii_weight_map(sf21_hii_ctx *ctx, int *host_out);
+/* Filtered pair of one ladder radius for a slice: out (device, 2*n^3
+ * floats, interleaved) = n^3 * (W_R rec, W_R ion) as the unnormalised c2r of
+ * get_HIIbubbles.c:324-325 gives them. */
+int sf21_hii_filtered_dev(sf21_hii_ctx *ctx, const float *a, const float *b, const float *c,
+                          double z, int radius_index, float *out);
+
+/* Kernel timing with HIP events on the context's stream. */
+int sf21_hii_profile(sf21_hii_ctx *ctx, int enable);
+int sf21_hii_kernel_stats(sf21_hii_ctx *ctx, int index, char *name, int name_cap,
+                          long *launches, double *total_ms);
+int sf21_hii_reset_stats(sf21_hii_ctx *ctx);
+
+/* ---- FFTW-like 3D transforms on device memory --------------------------
+ * Replace fftwf_plan_dft_r2c_3d / fftwf_plan_dft_c2r_3d / fftwf_execute as
+ * used at get_HIIbubbles.c:110, :128, :137, :146, :159, :176, :180, :290-291,
+ * :324-325, :372-373, :382.  Unnormalised, forward sign -1, half layout
+ * n x n x (n/2+1) interleaved complex.  C2R destroys its input, like FFTW's
+ * multi-dimensional c2r. */
+#define SF21_FFT_R2C 0
+#define SF21_FFT_C2R 1
+#define SF21_FFT_C2C_FORWARD 2
+#define SF21_FFT_C2C_BACKWARD 3
+typedef struct sf21_fft_plan sf21_fft_plan;
+int sf21_fft_plan_3d(int n, int kind, sf21_fft_plan **out);
+int sf21_fft_execute(sf21_fft_plan *plan, void *in, void *out, void *hip_stream);
+int sf21_fft_destroy(sf21_fft_plan *plan);
+
+/* ---- host logic (no GPU needed) ---------------------------------------- */
+
+/* The parameters get_HIIbubbles reads from base_dir/simfast21.ini, parsed as
+ * Input_variables.c:16-141 does, with its derived values. */
+typedef struct sf21_params {
+  int nthreads;
+  long n_halo, n;          /* N_halo, N_smoothed */
+  double zmin, zmax, dz;   /* zmin snapped to zmax - dz*ceil((zmax-zmin)/dz) (:133) */
+  double box_len;          /* sim_length * hubble, Mpc/h (:121) */
+  double hubble;
+  double fesc, ion_cutoff;
+  double bubble_rmax;      /* bubble_Rmax * hubble, capped at box_len/2 (:123-124) */
+  int bubble_nbins;
+  int use_sfr;             /* 0 unless use_SFR = T */
+  double dx, dx_halo, dk;  /* box_len/n, box_len/n_halo, 2*pi/box_len (:127-129) */
+} sf21_params;
+/* msg receives the reference's message for SF21_ERR_IO / SF21_ERR_FORMAT. */
+int sf21_params_read(const char *base_dir, sf21_params *out, char *msg, int msg_cap);
+
+/* Radius ladder of get_HIIbubbles.c:192, :298-299, :397, :400-402, :461:
+ * bfactor = 10^(log10(rmax/dx)/nbins), R divided by bfactor in FP64 while
+ * R >= 2dx (main), then while R >= dx (small).  Returns the count or
+ * SF21_ERR_ARG if cap is too small; *n_main = main-loop count. */
+int sf21_ladder(double rmax, double dx, int nbins, double *radii, int cap, int *n_main);
+/* Largest integer m with sqrt((double)m)*dx <= R: the ball bound of the
+ * tests at get_HIIbubbles.c:366, :443, :447. */
+long sf21_ball_d2max(double R, double dx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SF21_HII_H */

@@ -1,0 +1,217 @@
+// Device building blocks of the CDNA4 FFT used by the bubble finder.
+//
+// A line of length L is spread over TPL = L/E threads, each holding E
+// complex values in registers.  Stages are self-sorting Stockham passes
+// (radix R, span NS = product of the radices already applied).  For
+// butterfly j in [0, L/R):
+//     k   = j mod NS
+//     a_r = x[j + r*L/R] * exp(sign*2*pi*i*k*r/(NS*R))
+//     a   = DFT_R(a)
+//     y[(j/NS)*NS*R + k + r*NS] = a_r
+// Thread t owns butterflies j = t + TPL*b (b < E/R), which makes its input
+// slots "element t + TPL*m" for every stage; the last stage's outputs land
+// in the same slots, so loads and stores of a whole line are both in the
+// pattern element(t + TPL*m) -> coalesced for contiguous lines and, with
+// lanes mapped across columns, for strided ones.  Only the exchanges
+// between stages go through LDS (split re/im arrays, padded against bank
+// conflicts).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace sf21 {
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+// multiply by sign*i
+template <int SIGN>
+__device__ __forceinline__ float2 mul_si(float2 a) {
+  return SIGN > 0 ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+
+// In-register DFTs, y_k = sum_n x_n exp(SIGN*2*pi*i*n*k/R).
+template <int R, int SIGN>
+struct Dft;
+
+template <int SIGN>
+struct Dft<2, SIGN> {
+  static __device__ __forceinline__ void run(float2* a) {
+    float2 t = a[1];
+    a[1] = csub(a[0], t);
+    a[0] = cadd(a[0], t);
+  }
+};
+
+template <int SIGN>
+struct Dft<4, SIGN> {
+  static __device__ __forceinline__ void run(float2* a) {
+    float2 t0 = cadd(a[0], a[2]), t1 = csub(a[0], a[2]);
+    float2 t2 = cadd(a[1], a[3]), t3 = mul_si<SIGN>(csub(a[1], a[3]));
+    a[0] = cadd(t0, t2);
+    a[2] = csub(t0, t2);
+    a[1] = cadd(t1, t3);
+    a[3] = csub(t1, t3);
+  }
+};
+
+template <int SIGN>
+struct Dft<8, SIGN> {
+  static __device__ __forceinline__ void run(float2* a) {
+    const float h = 0.70710678118654752440f;
+    float2 e[4] = {a[0], a[2], a[4], a[6]};
+    float2 o[4] = {a[1], a[3], a[5], a[7]};
+    Dft<4, SIGN>::run(e);
+    Dft<4, SIGN>::run(o);
+    // o[k] *= exp(SIGN*2*pi*i*k/8)
+    float2 o1 = o[1], o3 = o[3];
+    o[1] = make_float2(h * (o1.x - SIGN * o1.y), h * (o1.y + SIGN * o1.x));
+    o[2] = mul_si<SIGN>(o[2]);
+    o[3] = make_float2(h * (-o3.x - SIGN * o3.y), h * (-o3.y + SIGN * o3.x));
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      a[k] = cadd(e[k], o[k]);
+      a[k + 4] = csub(e[k], o[k]);
+    }
+  }
+};
+
+template <int SIGN>
+struct Dft<3, SIGN> {
+  static __device__ __forceinline__ void run(float2* a) {
+    const float c = -0.5f, s = SIGN * 0.86602540378443864676f;
+    float2 t1 = cadd(a[1], a[2]), t2 = csub(a[1], a[2]);
+    float2 m = make_float2(a[0].x + c * t1.x, a[0].y + c * t1.y);
+    float2 r = make_float2(-s * t2.y, s * t2.x);
+    a[0] = cadd(a[0], t1);
+    a[1] = cadd(m, r);
+    a[2] = csub(m, r);
+  }
+};
+
+template <int SIGN>
+struct Dft<5, SIGN> {
+  static __device__ __forceinline__ void run(float2* a) {
+    const float c1 = 0.30901699437494742410f, c2 = -0.80901699437494742410f;
+    const float s1 = SIGN * 0.95105651629515357212f, s2 = SIGN * 0.58778525229247312917f;
+    float2 b1 = cadd(a[1], a[4]), b4 = csub(a[1], a[4]);
+    float2 b2 = cadd(a[2], a[3]), b3 = csub(a[2], a[3]);
+    float2 m1 = make_float2(a[0].x + c1 * b1.x + c2 * b2.x, a[0].y + c1 * b1.y + c2 * b2.y);
+    float2 m2 = make_float2(a[0].x + c2 * b1.x + c1 * b2.x, a[0].y + c2 * b1.y + c1 * b2.y);
+    // i*(s1*b4 + s2*b3) and i*(s2*b4 - s1*b3)
+    float2 n1 = make_float2(-(s1 * b4.y + s2 * b3.y), s1 * b4.x + s2 * b3.x);
+    float2 n2 = make_float2(-(s2 * b4.y - s1 * b3.y), s2 * b4.x - s1 * b3.x);
+    a[0] = cadd(a[0], cadd(b1, b2));
+    a[1] = cadd(m1, n1);
+    a[4] = csub(m1, n1);
+    a[2] = cadd(m2, n2);
+    a[3] = csub(m2, n2);
+  }
+};
+
+// LDS index padding: one slot every 8 breaks the stride-R write pattern of
+// the first stage.
+__device__ __forceinline__ int lds_pad(int i) { return i + (i >> 3); }
+template <int L>
+struct LdsLine {
+  static constexpr int kFloats = L + L / 8 + 1;  // odd stride between lines
+};
+
+// Radix plans.  E = values per thread; the radices all divide E.
+template <int L>
+struct Plan;
+template <> struct Plan<32>  { static constexpr int E = 8; };
+template <> struct Plan<48>  { static constexpr int E = 12; };
+template <> struct Plan<64>  { static constexpr int E = 8; };
+template <> struct Plan<96>  { static constexpr int E = 12; };
+template <> struct Plan<128> { static constexpr int E = 8; };
+template <> struct Plan<192> { static constexpr int E = 12; };
+template <> struct Plan<256> { static constexpr int E = 8; };
+template <> struct Plan<512> { static constexpr int E = 8; };
+
+template <int... Rs>
+struct RadixList {};
+
+template <int L> struct Radices;
+template <> struct Radices<32>  { using type = RadixList<8, 4>; };
+template <> struct Radices<48>  { using type = RadixList<4, 4, 3>; };
+template <> struct Radices<64>  { using type = RadixList<8, 8>; };
+template <> struct Radices<96>  { using type = RadixList<4, 4, 2, 3>; };
+template <> struct Radices<128> { using type = RadixList<8, 8, 2>; };
+template <> struct Radices<192> { using type = RadixList<4, 4, 4, 3>; };
+template <> struct Radices<256> { using type = RadixList<8, 8, 4>; };
+template <> struct Radices<512> { using type = RadixList<8, 8, 8>; };
+
+// One Stockham stage; recurses into the remaining radices.
+// tw: table exp(-2*pi*i*q/L), q < L.  sre/sim: this line's LDS region.
+template <int L, int E, int SIGN, int NS, int R, int... REST>
+__device__ __forceinline__ void stages(float2 (&v)[E], float* sre, float* sim, int t,
+                                       const float2* __restrict__ tw) {
+  constexpr int TPL = L / E;
+  constexpr int NB = E / R;
+  constexpr bool LAST = sizeof...(REST) == 0;
+  static_assert(E % R == 0, "radix must divide the per-thread count");
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const int j = t + TPL * b;
+    const int k = j % NS;
+    float2 a[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) a[r] = v[b + r * NB];
+    if constexpr (NS > 1) {
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        float2 w = tw[k * r * (L / (NS * R))];
+        if (SIGN > 0) w.y = -w.y;
+        a[r] = cmul(a[r], w);
+      }
+    }
+    Dft<R, SIGN>::run(a);
+    if constexpr (LAST) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[b + r * NB] = a[r];
+    } else {
+      const int base = (j / NS) * NS * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int o = lds_pad(base + r * NS);
+        sre[o] = a[r].x;
+        sim[o] = a[r].y;
+      }
+    }
+  }
+  if constexpr (!LAST) {
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const int o = lds_pad(t + TPL * m);
+      v[m] = make_float2(sre[o], sim[o]);
+    }
+    __syncthreads();
+    stages<L, E, SIGN, NS * R, REST...>(v, sre, sim, t, tw);
+  }
+}
+
+template <int L, int E, int SIGN, int... Rs>
+__device__ __forceinline__ void line_fft_impl(float2 (&v)[E], float* sre, float* sim, int t,
+                                              const float2* __restrict__ tw, RadixList<Rs...>) {
+  stages<L, E, SIGN, 1, Rs...>(v, sre, sim, t, tw);
+}
+
+// Full line transform.  Every thread of the block must call it (it contains
+// block barriers), including threads whose line is out of range.
+template <int L, int SIGN>
+__device__ __forceinline__ void line_fft(float2 (&v)[Plan<L>::E], float* sre, float* sim, int t,
+                                         const float2* __restrict__ tw) {
+  line_fft_impl<L, Plan<L>::E, SIGN>(v, sre, sim, t, tw, typename Radices<L>::type{});
+}
+
+// Signed frequency index as the reference folds it (get_HIIbubbles.c:309-315):
+// i > n/2 -> i - n; i = n/2 stays positive.  Only its square is used.
+__device__ __forceinline__ int fold_sq(int i, int n) {
+  int f = i > (n >> 1) ? i - n : i;
+  return f * f;
+}
+
+}  // namespace sf21

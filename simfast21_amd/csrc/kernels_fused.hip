@@ -1,0 +1,639 @@
+// Fused exact-path kernels of the bubble finder (get_HIIbubbles.c:216-479,
+// restructured for one GPU):
+//
+//   k_fwd_rows      per slice: cell pre-pass (:216-225, :269-274 or direct
+//                   boxes) fused into the forward transform along p of the
+//                   packed field z = rec + i*ion (both r2c of :290-291 in one
+//                   complex transform).
+//   k_cols          forward along j (per slice), inverse along j (per radius).
+//   k_filter_i      per radius batch: forward along i, then for each radius
+//                   the top-hat filter (:305-322) and the inverse along i.
+//   k_inv_rows_crit per radius: inverse along p and the bubble-centre test
+//                   (:327-352, :437-438); centres record the ball bound of
+//                   the radius in a per-cell max map.
+//   k_dt_*          exact periodic dilation of all centres by their balls at
+//                   once (replaces :358-393 and :428-458): a separable
+//                   max-plus parabola transform of the weight map; the last
+//                   pass is fused with the quick fill (:277-288), the clamp
+//                   (:466-471) and the FP64 mean (:473-477).
+#include "fft_core.h"
+#include "kernels.h"
+#include <climits>
+
+namespace sf21 {
+
+// ------------------------------------------------------------------------
+// Cell fields
+// ------------------------------------------------------------------------
+
+// rec, ion and residual neutral fraction of one cell.  SFR mode restates
+// get_HIIbubbles.c:220-222 and :273 in FP32: Rrec (user_functions.c:58-65)
+// as k_rec * (x/(1+x))^4 with x = (D/r2)^r3, D = fl32(1+delta); XHI
+// (user_functions.c:101-106) in its cancellation-free form
+// 2r/(2r+1+sqrt(4r+1)), algebraically identical to the reference's.
+__device__ __forceinline__ void cell_fields(const CellConsts& k, const float* __restrict__ a,
+                                            const float* __restrict__ b,
+                                            const float* __restrict__ c, long i, float& rec,
+                                            float& ion, float& fres) {
+  if (k.mode == 0) {
+    const float d = a[i];
+    const float D = 1.0f + d;
+    const float r = fmaf(d, k.c_fres, k.c_fres);
+    fres = 2.0f * r / (2.0f * r + 1.0f + sqrtf(4.0f * r + 1.0f));
+    const float x = powf(D / k.r2, k.r3);
+    const float y = x / (1.0f + x);
+    const float y2 = y * y;
+    rec = k.k_rec * (y2 * y2);
+    ion = b[i] * k.k_ion;
+  } else {
+    rec = a[i];
+    ion = b[i];
+    fres = c ? c[i] : 0.0f;
+  }
+}
+
+// get_HIIbubbles.c:281-288: partial ionisation of a cell from its own
+// emissivity and recombination rate.
+__device__ __forceinline__ float quick_fill(float rec, float ion) {
+  if (ion > 0.0f) {
+    if (rec > 0.0f) {
+      const float q = ion / rec;
+      return q >= 1.0f ? 1.0f : q;
+    }
+    return 1.0f;
+  }
+  return 0.0f;
+}
+
+// get_HIIbubbles.c:339-349: centre iff ion > 0 and (rec <= 0 or
+// ion/rec >= 1); for rec > 0 the FP64 quotient test equals ion >= rec.
+__device__ __forceinline__ bool is_centre(float rec, float ion) {
+  return (ion > 0.0f) && (!(rec > 0.0f) || ion >= rec);
+}
+
+// ------------------------------------------------------------------------
+// Channel balance of the packed transform.  rec and ion share one complex
+// transform (z = rec + i*s*ion); FP32 rounding of a complex FFT is spread
+// over both channels in proportion to |z|, so without balancing the smaller
+// field inherits the larger one's absolute error.  s = 2^round(log2(
+// rms(rec)/rms(ion))) gives both channels the same RMS; a power of two keeps
+// the scaling exact (the centre test becomes Im >= s*Re, and Im/s is the
+// unscaled ion field bit for bit).  Sums in FP64.
+// ------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_balance_partial(long n3, const float* __restrict__ a,
+                                                         const float* __restrict__ b,
+                                                         const float* __restrict__ c, CellConsts k,
+                                                         double* __restrict__ part) {
+  double sr = 0.0, si = 0.0;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n3;
+       i += (long)gridDim.x * blockDim.x) {
+    float rec, ion, fr;
+    cell_fields(k, a, b, c, i, rec, ion, fr);
+    sr += (double)rec * rec;
+    si += (double)ion * ion;
+  }
+  __shared__ double r1[256], r2[256];
+  r1[threadIdx.x] = sr;
+  r2[threadIdx.x] = si;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      r1[threadIdx.x] += r1[threadIdx.x + s];
+      r2[threadIdx.x] += r2[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = r1[0];
+    part[2 * blockIdx.x + 1] = r2[0];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_balance_final(const double* __restrict__ part, int np,
+                                                       float* __restrict__ scale) {
+  __shared__ double r1[256], r2[256];
+  double sr = 0.0, si = 0.0;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) {
+    sr += part[2 * i];
+    si += part[2 * i + 1];
+  }
+  r1[threadIdx.x] = sr;
+  r2[threadIdx.x] = si;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      r1[threadIdx.x] += r1[threadIdx.x + s];
+      r2[threadIdx.x] += r2[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    int e = 0;
+    if (r1[0] > 0.0 && r2[0] > 0.0) {
+      e = (int)rint(0.5 * log2(r1[0] / r2[0]));
+      e = e < -60 ? -60 : (e > 60 ? 60 : e);
+    }
+    scale[0] = ldexpf(1.0f, e);
+  }
+}
+
+// ------------------------------------------------------------------------
+// Forward along p with the fused cell pre-pass.
+// ------------------------------------------------------------------------
+template <int L>
+__global__ __launch_bounds__(256) void k_fwd_rows(const float* __restrict__ a,
+                                                  const float* __restrict__ b,
+                                                  const float* __restrict__ c, CellConsts k,
+                                                  const float* __restrict__ bal,
+                                                  float2* __restrict__ y,
+                                                  const float2* __restrict__ tw) {
+  constexpr int E = Plan<L>::E, TPL = L / E, LPB = 256 / TPL;
+  __shared__ float lds[LPB * 2 * LdsLine<L>::kFloats];
+  const int lw = threadIdx.x / TPL, t = threadIdx.x % TPL;
+  const long row = (long)blockIdx.x * LPB + lw;
+  float* sre = lds + lw * 2 * LdsLine<L>::kFloats;
+  float* sim = sre + LdsLine<L>::kFloats;
+  const float sc = bal[0];
+  float2 v[E];
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    float rec, ion, fr;
+    cell_fields(k, a, b, c, row * L + t + TPL * m, rec, ion, fr);
+    v[m] = make_float2(rec, ion * sc);
+  }
+  line_fft<L, -1>(v, sre, sim, t, tw);
+#pragma unroll
+  for (int m = 0; m < E; ++m) y[row * L + t + TPL * m] = v[m];
+}
+
+// ------------------------------------------------------------------------
+// Strided complex transform along j (axis 0) or i (axis 1) of an
+// L x L x width array; TILE adjacent columns per block (128 B runs).
+// blockIdx.y selects one of several arrays spaced bstride apart.
+// ------------------------------------------------------------------------
+template <int L, int SIGN, int TILE>
+__global__ __launch_bounds__(TILE*(L / Plan<L>::E)) void k_cols(const float2* in, float2* out,
+                                                                int width, int axis,
+                                                                long bstride,
+                                                                const float2* __restrict__ tw) {
+  constexpr int E = Plan<L>::E, TPL = L / E;
+  __shared__ float lds[TILE * 2 * LdsLine<L>::kFloats];
+  const int c = threadIdx.x % TILE, t = threadIdx.x / TILE;
+  const int ntile = (width + TILE - 1) / TILE;
+  const int outer = blockIdx.x / ntile;
+  const int col = (blockIdx.x % ntile) * TILE + c;
+  const bool ok = col < width;
+  long base, stride;
+  if (axis == 0) {
+    base = (long)outer * L * width + col;
+    stride = width;
+  } else {
+    base = (long)outer * width + col;
+    stride = (long)L * width;
+  }
+  base += (long)blockIdx.y * bstride;
+  float* sre = lds + c * 2 * LdsLine<L>::kFloats;
+  float* sim = sre + LdsLine<L>::kFloats;
+  float2 v[E];
+  if (ok) {
+#pragma unroll
+    for (int m = 0; m < E; ++m) v[m] = in[base + (long)(t + TPL * m) * stride];
+  } else {
+#pragma unroll
+    for (int m = 0; m < E; ++m) v[m] = make_float2(0.f, 0.f);
+  }
+  line_fft<L, SIGN>(v, sre, sim, t, tw);
+  if (ok) {
+#pragma unroll
+    for (int m = 0; m < E; ++m) out[base + (long)(t + TPL * m) * stride] = v[m];
+  }
+}
+
+// ------------------------------------------------------------------------
+// Per radius batch: forward along i of the (p, j)-transformed spectrum,
+// top-hat filter W(|k| R) for each radius (get_HIIbubbles.c:305-322; W from
+// an FP64 table indexed by the integer |k|^2, rounded to float), inverse
+// along i.  The forward spectrum along i never leaves the registers.
+// ------------------------------------------------------------------------
+template <int L, int TILE>
+__global__ __launch_bounds__(TILE*(L / Plan<L>::E)) void k_filter_i(
+    const float2* __restrict__ spec, float2* __restrict__ u, int nrad, long ustride,
+    const float* __restrict__ wlut, long mlut, const float2* __restrict__ tw) {
+  constexpr int E = Plan<L>::E, TPL = L / E, NT = L / TILE;
+  __shared__ float lds[TILE * 2 * LdsLine<L>::kFloats];
+  const int c = threadIdx.x % TILE, t = threadIdx.x / TILE;
+  const int kj = blockIdx.x / NT;
+  const int kp = (blockIdx.x % NT) * TILE + c;
+  const long base = (long)kj * L + kp, stride = (long)L * L;
+  float* sre = lds + c * 2 * LdsLine<L>::kFloats;
+  float* sim = sre + LdsLine<L>::kFloats;
+  float2 z[E];
+#pragma unroll
+  for (int m = 0; m < E; ++m) z[m] = spec[base + (long)(t + TPL * m) * stride];
+  line_fft<L, -1>(z, sre, sim, t, tw);
+  const int mjp = fold_sq(kj, L) + fold_sq(kp, L);
+  int ms[E];
+#pragma unroll
+  for (int m = 0; m < E; ++m) ms[m] = mjp + fold_sq(t + TPL * m, L);
+#pragma unroll 1
+  for (int r = 0; r < nrad; ++r) {
+    const float* __restrict__ w = wlut + (long)r * mlut;
+    float2 v[E];
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const float wv = w[ms[m]];
+      v[m] = make_float2(z[m].x * wv, z[m].y * wv);
+    }
+    line_fft<L, +1>(v, sre, sim, t, tw);
+    float2* __restrict__ ur = u + (long)r * ustride;
+#pragma unroll
+    for (int m = 0; m < E; ++m) ur[base + (long)(t + TPL * m) * stride] = v[m];
+  }
+}
+
+// ------------------------------------------------------------------------
+// Per radius: inverse along p -> (N^3 rec_R, N^3 ion_R), centre test, and
+// wmap[cell] = max(wmap[cell], d2max[r]) for centres.  blockIdx.y = radius
+// within the batch.  dbg (nullable) receives the filtered pair.
+// ------------------------------------------------------------------------
+template <int L>
+__global__ __launch_bounds__(256) void k_inv_rows_crit(const float2* __restrict__ vin,
+                                                       long vstride,
+                                                       const int* __restrict__ d2max,
+                                                       const float* __restrict__ bal,
+                                                       int* __restrict__ wmap,
+                                                       float2* __restrict__ dbg,
+                                                       const float2* __restrict__ tw) {
+  constexpr int E = Plan<L>::E, TPL = L / E, LPB = 256 / TPL;
+  __shared__ float lds[LPB * 2 * LdsLine<L>::kFloats];
+  const int lw = threadIdx.x / TPL, t = threadIdx.x % TPL;
+  const long row = (long)blockIdx.x * LPB + lw;
+  const int r = blockIdx.y;
+  const float2* __restrict__ src = vin + (long)r * vstride;
+  float* sre = lds + lw * 2 * LdsLine<L>::kFloats;
+  float* sim = sre + LdsLine<L>::kFloats;
+  float2 v[E];
+#pragma unroll
+  for (int m = 0; m < E; ++m) v[m] = src[row * L + t + TPL * m];
+  line_fft<L, +1>(v, sre, sim, t, tw);
+  const int w = d2max[r];
+  const float sc = bal[0];
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const long idx = row * L + t + TPL * m;
+    // Re = N^3 rec_R, Im = s N^3 ion_R: centre test on (s rec_R, s ion_R)
+    if (is_centre(v[m].x * sc, v[m].y)) atomicMax(&wmap[idx], w);
+    if (dbg) dbg[(long)r * vstride + idx] = make_float2(v[m].x, v[m].y / sc);
+  }
+}
+
+__global__ void k_fill_i32(int* __restrict__ p, int value, long count) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+       i += (long)gridDim.x * blockDim.x)
+    p[i] = value;
+}
+
+// ------------------------------------------------------------------------
+// Exact dilation: g(x) = max_y (w(y) - minimg|x-y|^2) over cells with
+// w(y) >= 0, computed one axis at a time (the max-plus transform with a
+// parabola is separable).  A cell is ionised iff g(x) >= 0, i.e. iff some
+// centre's ball {|d|^2 <= d2max(R)} covers it.  Negative intermediates can
+// never become non-negative again and are stored as -1.
+//
+// 1D pass over a periodic line of n values, one wavefront per line, the line
+// in LDS.  On the unrolled line, h(x, y) = f(y) - (x-y)^2 is supermodular
+// (h(x1,y1) + h(x2,y2) - h(x1,y2) - h(x2,y1) = 2(x2-x1)(y2-y1) > 0), so the
+// maximisers of x are non-decreasing in x.  Outputs are found in three
+// levels: every 8*Q-th x by an outward scan with early exit (stop once
+// d^2 > fmax - best), every Q-th x inside the bracket of its level-0
+// neighbours, the rest inside the bracket of their level-1 neighbours.  A
+// neighbour xa that found no non-negative value bounds the search too: a
+// y < xa with f(y) >= (x-y)^2 > (xa-y)^2 would contradict it.  Only values
+// >= 0 are ever needed, so candidates farther than floor(sqrt(fmax)) are
+// never visited.
+// ------------------------------------------------------------------------
+struct Found {
+  int best;  // -1: nothing >= 0
+  int lo, hi;
+};
+
+template <class F>
+__device__ __forceinline__ void consider(int x, int y, F f, Found& r) {
+  const int v = f(y);
+  if (v < 0) return;
+  const int d = x - y;
+  const int c = v - d * d;
+  if (c > r.best) {
+    r.best = c;
+    r.lo = y;
+    r.hi = y;
+  } else if (c == r.best) {
+    r.lo = min(r.lo, y);
+    r.hi = max(r.hi, y);
+  }
+}
+
+template <class F>
+__device__ __forceinline__ Found scan_out(int x, int hl, int fmax, F f) {
+  Found r{-1, 0, 0};
+  for (int d = 0; d <= hl; ++d) {
+    if (r.best >= 0 && d * d > fmax - r.best) break;
+    consider(x, x - d, f, r);
+    if (d) consider(x, x + d, f, r);
+  }
+  return r;
+}
+
+template <class F>
+__device__ __forceinline__ Found scan_range(int x, int lo, int hi, int hl, F f) {
+  Found r{-1, 0, 0};
+  lo = max(lo, x - hl);
+  hi = min(hi, x + hl);
+  for (int y = lo; y <= hi; ++y) consider(x, y, f, r);
+  return r;
+}
+
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+__device__ __forceinline__ int isqrt_floor(int v) {
+  int r = (int)sqrtf((float)v);
+  while (r * r > v) --r;
+  while ((r + 1) * (r + 1) <= v) ++r;
+  return r;
+}
+
+// Transform one line held in LDS (element e at col[e + (e >> 3)]); each of
+// the P1 = n/Q active lanes returns the outputs x = Q*lane + q, q < Q.
+template <int QMAX>
+__device__ __forceinline__ void dt_line_wave(const int* col, int n, int Q, int hcap, int lane,
+                                             int (&out)[QMAX]) {
+  const int P1 = n / Q;
+  const bool act = lane < P1;
+  auto f = [&](int y) {
+    y = y < 0 ? y + n : (y >= n ? y - n : y);
+    return col[y + (y >> 3)];
+  };
+  int lmax = -1;
+  if (act)
+    for (int q = 0; q < Q; ++q) lmax = max(lmax, col[Q * lane + q + ((Q * lane + q) >> 3)]);
+  const int fmax = wave_max(lmax);
+  if (fmax < 0) {
+#pragma unroll
+    for (int q = 0; q < QMAX; ++q) out[q] = -1;
+    return;
+  }
+  const int hl = min(hcap, isqrt_floor(fmax));
+  const int x1 = Q * lane;
+  // level 0: lanes 0, 8, 16, ... (x = 8*Q*m)
+  Found l0{-1, 0, 0};
+  if (act && (lane & 7) == 0) l0 = scan_out(x1, hl, fmax, f);
+  // level 1: bracket [xa, xb] of level-0 points, xb = xa + 8Q (wrapping to 0 + n)
+  const int ga = lane & ~7;
+  const int gb = ga + 8;
+  const int wb = gb < P1 ? gb : 0;
+  const int a_best = __shfl(l0.best, ga, 64), a_lo = __shfl(l0.lo, ga, 64);
+  const int b_best = __shfl(l0.best, wb, 64), b_hi0 = __shfl(l0.hi, wb, 64);
+  Found l1 = l0;
+  if (act && (lane & 7) != 0) {
+    const int xa = Q * ga, xb = Q * gb;
+    const int lo = a_best >= 0 ? a_lo : xa;
+    const int hi = b_best >= 0 ? b_hi0 + (gb < P1 ? 0 : n) : xb;
+    l1 = scan_range(x1, lo, hi, hl, f);
+  }
+  // level 2: between x1 and the next lane's level-1 point
+  const int nl = lane + 1 < P1 ? lane + 1 : 0;
+  const int n_best = __shfl(l1.best, nl, 64), n_hi0 = __shfl(l1.hi, nl, 64);
+  out[0] = l1.best;
+#pragma unroll
+  for (int q = 1; q < QMAX; ++q) {
+    out[q] = -1;
+    if (act && q < Q) {
+      const int lo = l1.best >= 0 ? l1.lo : x1;
+      const int hi = n_best >= 0 ? n_hi0 + (lane + 1 < P1 ? 0 : n) : x1 + Q;
+      out[q] = scan_range(x1 + q, lo, hi, hl, f).best;
+    }
+  }
+}
+
+constexpr int kDtCols = 16;  // columns per block in the strided passes
+constexpr int kQmax = 8;     // n <= 512
+
+__host__ __device__ constexpr int dt_col_floats(int n) { return n + n / 8 + 1; }
+
+// Strided pass: lines along i (axis 1: outer = j) or j (axis 0: outer = i)
+// for a tile of kDtCols adjacent p columns; in place through LDS.
+__global__ __launch_bounds__(256) void k_dt_tile(DtArgs a, int axis, const int* in, int* out) {
+  extern __shared__ int sm[];
+  const int n = a.n, Q = (n + 63) / 64, cs = dt_col_floats(n);
+  const int ntile = n / kDtCols;
+  const int outer = blockIdx.x / ntile, p0 = (blockIdx.x % ntile) * kDtCols;
+  long base, stride;
+  if (axis == 0) {
+    base = (long)outer * n * n + p0;
+    stride = n;
+  } else {
+    base = (long)outer * n + p0;
+    stride = (long)n * n;
+  }
+  for (int t = threadIdx.x; t < n * kDtCols; t += blockDim.x) {
+    const int e = t / kDtCols, c = t % kDtCols;
+    sm[c * cs + e + (e >> 3)] = in[base + e * stride + c];
+  }
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int c = wave; c < kDtCols; c += blockDim.x >> 6) {
+    int* col = sm + c * cs;
+    int o[kQmax];
+    dt_line_wave<kQmax>(col, n, Q, a.h, lane, o);
+    if (lane < n / Q)
+      for (int q = 0; q < Q; ++q) {
+        const int e = Q * lane + q;
+        col[e + (e >> 3)] = o[q];
+      }
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < n * kDtCols; t += blockDim.x) {
+    const int e = t / kDtCols, c = t % kDtCols;
+    out[base + e * stride + c] = sm[c * cs + e + (e >> 3)];
+  }
+}
+
+// Pass along p, one row per wave, fused with the final per-cell value
+// (quick fill :277-288, union of balls, clamp :466-471) and FP64 partial
+// sums of 1 - xHII (:473-477).
+__global__ __launch_bounds__(256) void k_dt_rows_final(DtArgs a, const int* in,
+                                                       const float* __restrict__ pa,
+                                                       const float* __restrict__ pb,
+                                                       const float* __restrict__ pc, CellConsts k,
+                                                       float* __restrict__ xhii,
+                                                       double* __restrict__ partials) {
+  extern __shared__ int sm[];
+  const int n = a.n, Q = (n + 63) / 64, cs = dt_col_floats(n);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+  int* col = sm + wave * cs;
+  double acc = 0.0;
+  const long rows = (long)n * n;
+  // block-uniform trip count: every wave reaches both barriers each time
+  for (long r0 = (long)blockIdx.x * nw; r0 < rows; r0 += (long)gridDim.x * nw) {
+    const long row = r0 + wave;
+    const bool valid = row < rows;
+    if (valid) {
+      const int* src = in + row * n;
+      for (int e = lane; e < n; e += 64) col[e + (e >> 3)] = src[e];
+    }
+    __syncthreads();
+    int o[kQmax];
+    if (valid) dt_line_wave<kQmax>(col, n, Q, a.h, lane, o);
+    if (valid && lane < n / Q)
+      for (int q = 0; q < Q; ++q) {
+        const long idx = row * n + Q * lane + q;
+        float rec, ion, fres;
+        cell_fields(k, pa, pb, pc, idx, rec, ion, fres);
+        float v = o[q] >= 0 ? 1.0f : quick_fill(rec, ion);
+        v = fminf(v, 1.0f - fres);
+        xhii[idx] = v;
+        acc += 1.0 - (double)v;
+      }
+    __syncthreads();
+  }
+  __shared__ double red[256];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_sum_partials(const double* __restrict__ partials,
+                                                      int np, double scale,
+                                                      double* __restrict__ out) {
+  __shared__ double red[256];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) acc += partials[i];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = red[0] * scale;
+}
+
+// ------------------------------------------------------------------------
+// Launchers
+// ------------------------------------------------------------------------
+#define SF21_SIZES(X) X(32) X(64) X(96) X(128) X(192) X(256) X(512)
+
+int balance_partials_count(long n3) {
+  long b = (n3 + 255) / 256;
+  return (int)(b < 1024 ? (b < 1 ? 1 : b) : 1024);
+}
+
+void launch_balance(long n3, const float* a, const float* b, const float* c, CellConsts k,
+                    double* part, float* scale, hipStream_t s) {
+  const int nb = balance_partials_count(n3);
+  hipLaunchKernelGGL(k_balance_partial, dim3(nb), dim3(256), 0, s, n3, a, b, c, k, part);
+  hipLaunchKernelGGL(k_balance_final, dim3(1), dim3(256), 0, s, part, nb, scale);
+}
+
+void launch_fwd_rows(int n, const float* a, const float* b, const float* c, CellConsts k,
+                     const float* bal, float2* y, const float2* tw, hipStream_t s) {
+#define X(L)                                                                          \
+  if (n == L) {                                                                       \
+    constexpr int LPB = 256 / (L / Plan<L>::E);                                       \
+    hipLaunchKernelGGL(k_fwd_rows<L>, dim3((L * L) / LPB), dim3(256), 0, s, a, b, c, k, bal, y, tw); \
+    return;                                                                           \
+  }
+  SF21_SIZES(X)
+#undef X
+}
+
+constexpr int kTile = 16;
+
+void launch_cols(int n, int width, int axis, int sign, const float2* in, float2* out,
+                 int nbatch, long bstride, const float2* tw, hipStream_t s) {
+#define X(L)                                                                              \
+  if (n == L) {                                                                           \
+    constexpr int NT = kTile * (L / Plan<L>::E);                                          \
+    dim3 g(L * ((width + kTile - 1) / kTile), nbatch);                                    \
+    if (sign < 0)                                                                         \
+      hipLaunchKernelGGL((k_cols<L, -1, kTile>), g, dim3(NT), 0, s, in, out, width, axis, bstride, tw); \
+    else                                                                                  \
+      hipLaunchKernelGGL((k_cols<L, +1, kTile>), g, dim3(NT), 0, s, in, out, width, axis, bstride, tw); \
+    return;                                                                               \
+  }
+  SF21_SIZES(X)
+#undef X
+}
+
+void launch_filter_i(int n, const float2* spec, float2* u, int nrad, long ustride,
+                     const float* wlut, long m_lut, const float2* tw, hipStream_t s, int tile) {
+#define X(L, TL)                                                                          \
+  if (n == L && tile == TL) {                                                             \
+    constexpr int NT = TL * (L / Plan<L>::E);                                             \
+    hipLaunchKernelGGL((k_filter_i<L, TL>), dim3(L * (L / TL)), dim3(NT), 0, s, spec, u, nrad, \
+                       ustride, wlut, m_lut, tw);                                         \
+    return;                                                                               \
+  }
+#define Y(L) X(L, 4) X(L, 8) X(L, 16)
+  Y(32) Y(64) Y(96) Y(128) Y(192) Y(256) Y(512)
+#undef Y
+#undef X
+}
+
+void launch_inv_rows_crit(int n, const float2* v, int nrad, long vstride, const int* d2max,
+                          const float* bal, int* wmap, float2* dbg, hipStream_t s,
+                          const float2* tw) {
+#define X(L)                                                                              \
+  if (n == L) {                                                                           \
+    constexpr int LPB = 256 / (L / Plan<L>::E);                                           \
+    hipLaunchKernelGGL(k_inv_rows_crit<L>, dim3((L * L) / LPB, nrad), dim3(256), 0, s, v, vstride, \
+                       d2max, bal, wmap, dbg, tw);                                        \
+    return;                                                                               \
+  }
+  SF21_SIZES(X)
+#undef X
+}
+
+void launch_fill_i32(int* p, int value, long count, hipStream_t s) {
+  long blocks = (count + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)blocks), dim3(256), 0, s, p, value, count);
+}
+
+int dt_partials_count(int n) {
+  const long rows = (long)n * n;
+  const long blocks = (rows + 3) / 4;
+  return (int)(blocks < 2048 ? blocks : 2048);
+}
+
+void launch_dt_rows(const DtArgs& a, const int* in, int* out, hipStream_t s) {
+  // axis i: tiles (j, p-block)
+  const size_t lds = sizeof(int) * kDtCols * dt_col_floats(a.n);
+  hipLaunchKernelGGL(k_dt_tile, dim3(a.n * (a.n / kDtCols)), dim3(256), lds, s, a, 1, in, out);
+}
+void launch_dt_cols(const DtArgs& a, const int* in, int* out, hipStream_t s) {
+  // axis j: tiles (i, p-block)
+  const size_t lds = sizeof(int) * kDtCols * dt_col_floats(a.n);
+  hipLaunchKernelGGL(k_dt_tile, dim3(a.n * (a.n / kDtCols)), dim3(256), lds, s, a, 0, in, out);
+}
+void launch_dt_final(const DtArgs& a, const int* in, const float* pa, const float* pb,
+                     const float* pc, CellConsts k, float* xhii, double* partials,
+                     int npartials, hipStream_t s) {
+  const size_t lds = sizeof(int) * 4 * dt_col_floats(a.n);
+  hipLaunchKernelGGL(k_dt_rows_final, dim3(npartials), dim3(256), lds, s, a, in, pa, pb, pc, k,
+                     xhii, partials);
+}
+void launch_sum_partials(const double* partials, int np, double scale, double* out,
+                         hipStream_t s) {
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, partials, np, scale, out);
+}
+
+}  // namespace sf21

@@ -1,0 +1,705 @@
+// libsf21hii: context, launch sequences and C-ABI of the MI355X bubble
+// finder.  See include/sf21_hii.h for the contract.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/sf21_hii.h"
+#include "kernels.h"
+
+using namespace sf21;
+
+namespace {
+
+// ---- astrophysical fits of the SFR-mode pre-pass (user_functions.c) ----
+constexpr double kCmPerMpc = 3.0857e24;  // user_functions.c:26
+constexpr double kVnorm = 1.e+45;        // user_functions.c:38
+constexpr double kR1 = 9.84815696, kR2 = 1.76105133, kR3 = 0.81722878, kR4 = 5.06773134;
+constexpr double kQa = 2.05707274e-02, kQb = 5.29944413e+01;
+constexpr double kPi = 3.1415926535897932385E0;  // Input_variables.h:10
+
+// user_functions.c:92-97
+double fit_G_H(double z) {
+  const double e = -4.37152818e-05 * std::pow(z, 5.) + 1.64394635e-03 * std::pow(z, 4.) -
+                   2.03044756e-02 * std::pow(z, 3.) + 7.18335221e-02 * std::pow(z, 2.) -
+                   8.85791388e-02 * z - 1.20887500e+01;
+  return 4.19232273531e-13 / std::pow(10.0, e);
+}
+// user_functions.c:84-88
+double fit_Qion(double z) { return std::pow(10.0, kQa * z + kQb) / kVnorm; }
+
+// auxiliary.c:350-359
+double w_filter(double x) {
+  double j;
+  if (x < 0.03)
+    j = 1.0 / 3.0 - x * x / 30.0;
+  else
+    j = -(std::cos(x) - std::sin(x) / x) / x / x;
+  return 3.0 * j;
+}
+
+bool size_supported(int n, int path) {
+  switch (n) {
+    case 32: return path == SF21_PATH_FUSED;
+    case 64: case 96: case 128: case 192: case 256: case 512: return true;
+    default: return false;
+  }
+}
+
+struct KStat {
+  std::string name;
+  long launches = 0;
+  double ms = 0.0;
+};
+
+struct Pending {
+  int stat;
+  hipEvent_t a, b;
+};
+
+struct GraphEntry {
+  std::vector<const void*> ptrs;
+  std::vector<double> zs;
+  hipGraphExec_t exec = nullptr;
+};
+
+}  // namespace
+
+struct sf21_hii_ctx {
+  sf21_hii_config cfg{};
+  std::vector<double> radii;
+  std::vector<int> d2;
+  int n = 0, h = 0, K = 0;
+  int tile_filter = 8;
+  long n3 = 0, nh = 0;
+  double L = 0, dx = 0, dk = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  // tables
+  float2 *tw = nullptr, *twh = nullptr;
+  float* wlut = nullptr;
+  long mlut = 0;
+  int* d_d2 = nullptr;
+  // fused path
+  float2 *spec = nullptr, *ubuf = nullptr;
+  int *wmap = nullptr, *gbuf = nullptr;
+  double* bal_part = nullptr;
+  float* bal = nullptr;
+  DtArgs dt{};
+  double* partials = nullptr;
+  int npart = 0;
+  double* neutral_dev = nullptr;
+  int neutral_cap = 0;
+  double* neutral_host = nullptr;
+  // plain path
+  float *rec = nullptr, *ion = nullptr, *fres = nullptr, *bubblef = nullptr;
+  float *box_a = nullptr, *box_b = nullptr, *bubble = nullptr;
+  float2 *spec2 = nullptr, *tmp2 = nullptr, *th = nullptr;
+  int* flag_dev = nullptr;
+  int* flag_host = nullptr;
+  // profiling
+  bool prof = false;
+  std::vector<KStat> stats;
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> pool;
+  // graphs
+  std::vector<GraphEntry> graphs;
+  bool capturing = false;
+};
+
+struct sf21_fft_plan {
+  int n = 0, kind = 0;
+  float2 *tw = nullptr, *twh = nullptr;
+};
+
+namespace {
+
+template <class T>
+int dev_alloc(T** p, size_t count) {
+  if (hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T)) != hipSuccess) {
+    *p = nullptr;
+    (void)hipGetLastError();
+    return SF21_ERR_ALLOC;
+  }
+  return SF21_OK;
+}
+
+template <class T>
+void dev_free(T*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+// exp(-2 pi i q/len), q < len, rounded from FP64.
+int make_twiddles(int len, float2** out) {
+  std::vector<float2> h(len);
+  for (int q = 0; q < len; ++q) {
+    const double a = -2.0 * kPi * (double)q / (double)len;
+    h[q] = make_float2((float)std::cos(a), (float)std::sin(a));
+  }
+  int rc = dev_alloc(out, (size_t)len);
+  if (rc) return rc;
+  if (hipMemcpy(*out, h.data(), sizeof(float2) * len, hipMemcpyHostToDevice) != hipSuccess)
+    return SF21_ERR_HIP;
+  return SF21_OK;
+}
+
+// Per-slice constants of the SFR-mode pre-pass (get_HIIbubbles.c:220-222,
+// :273 with user_functions.c:58-65, :84-97), FP64 on the host.
+CellConsts cell_consts(const sf21_hii_ctx* c, double z) {
+  CellConsts k{};
+  k.mode = c->cfg.input_mode == SF21_INPUT_SFR ? 0 : 1;
+  if (k.mode == 0) {
+    const double h = c->cfg.hubble;
+    k.c_fres = (float)(1.881e-7 * std::pow(1. + z, 3.0) * fit_G_H(z));
+    const double dvv = std::pow(c->L * kCmPerMpc / c->n / h / (1. + z), 3.0);
+    k.k_rec = (float)(1.e-24 * kR1 * std::pow(1.0 + z, kR4) * dvv / kVnorm);
+    k.r2 = (float)kR2;
+    k.r3 = (float)kR3;
+    k.k_ion = (float)(c->dx * c->dx * c->dx * fit_Qion(z) * c->cfg.fesc);
+  }
+  return k;
+}
+
+// ---- profiling: HIP event pairs around each launch on the ctx stream ----
+int stat_index(sf21_hii_ctx* c, const char* name) {
+  for (size_t i = 0; i < c->stats.size(); ++i)
+    if (c->stats[i].name == name) return (int)i;
+  c->stats.push_back(KStat{name, 0, 0.0});
+  return (int)c->stats.size() - 1;
+}
+
+hipEvent_t get_event(sf21_hii_ctx* c) {
+  if (!c->pool.empty()) {
+    hipEvent_t e = c->pool.back();
+    c->pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  (void)hipEventCreate(&e);
+  return e;
+}
+
+template <class F>
+void launch(sf21_hii_ctx* c, const char* name, F&& f) {
+  if (c->prof && !c->capturing) {
+    Pending p{stat_index(c, name), get_event(c), get_event(c)};
+    (void)hipEventRecord(p.a, c->stream);
+    f();
+    (void)hipEventRecord(p.b, c->stream);
+    c->pending.push_back(p);
+  } else {
+    f();
+  }
+}
+
+void collect(sf21_hii_ctx* c) {
+  for (auto& p : c->pending) {
+    float ms = 0.f;
+    (void)hipEventSynchronize(p.b);
+    if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+      c->stats[p.stat].ms += ms;
+      c->stats[p.stat].launches += 1;
+    }
+    c->pool.push_back(p.a);
+    c->pool.push_back(p.b);
+  }
+  c->pending.clear();
+}
+
+int check_hip() {
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? SF21_OK : SF21_ERR_HIP;
+}
+
+// ---- fused exact path: enqueue one slice -------------------------------
+void enqueue_fused(sf21_hii_ctx* c, const float* a, const float* b, const float* cc, double z,
+                   float* xhii, double* neutral_slot, int only_radius, float2* dbg) {
+  const int n = c->n;
+  const long n3 = c->n3;
+  const CellConsts k = cell_consts(c, z);
+  hipStream_t s = c->stream;
+  launch(c, "balance", [&] { launch_balance(n3, a, b, cc, k, c->bal_part, c->bal, s); });
+  launch(c, "fwd_rows", [&] { launch_fwd_rows(n, a, b, cc, k, c->bal, c->spec, c->tw, s); });
+  launch(c, "cols_fwd_j", [&] { launch_cols(n, n, 0, -1, c->spec, c->spec, 1, 0, c->tw, s); });
+  launch(c, "fill_map", [&] { launch_fill_i32(c->wmap, -1, n3, s); });
+  const int r_begin = only_radius >= 0 ? only_radius : 0;
+  const int r_end = only_radius >= 0 ? only_radius + 1 : (int)c->radii.size();
+  for (int rb = r_begin; rb < r_end; rb += c->K) {
+    const int kk = std::min(c->K, r_end - rb);
+    launch(c, "filter_i", [&] {
+      launch_filter_i(n, c->spec, c->ubuf, kk, n3, c->wlut + (long)rb * c->mlut, c->mlut, c->tw,
+                      s, c->tile_filter);
+    });
+    launch(c, "cols_inv_j",
+           [&] { launch_cols(n, n, 0, +1, c->ubuf, c->ubuf, kk, n3, c->tw, s); });
+    launch(c, "inv_rows_crit", [&] {
+      launch_inv_rows_crit(n, c->ubuf, kk, n3, c->d_d2 + rb, c->bal, c->wmap, dbg, s, c->tw);
+    });
+  }
+  if (only_radius >= 0) return;
+  // the centre map stays intact for inspection; each line pass reads its
+  // whole line before writing it, so the later passes run in place
+  launch(c, "dt_rows", [&] { launch_dt_rows(c->dt, c->wmap, c->gbuf, s); });
+  launch(c, "dt_cols", [&] { launch_dt_cols(c->dt, c->gbuf, c->gbuf, s); });
+  launch(c, "dt_final", [&] {
+    launch_dt_final(c->dt, c->gbuf, a, b, cc, k, xhii, c->partials, c->npart, s);
+  });
+  launch(c, "sum", [&] {
+    launch_sum_partials(c->partials, c->npart, 1.0 / (double)n3, neutral_slot, s);
+  });
+}
+
+// ---- plain path helpers: FFTW-like 3D real transforms ------------------
+void r2c3d(sf21_hii_ctx* c, const float* in, float2* out) {
+  const int n = c->n, h = n / 2 + 1;
+  hipStream_t s = c->stream;
+  launch(c, "r2c_rows", [&] { launch_r2c_rows(n, in, out, c->twh, c->tw, s); });
+  launch(c, "cols_half", [&] { launch_cols(n, h, 0, -1, out, out, 1, 0, c->tw, s); });
+  launch(c, "cols_half", [&] { launch_cols(n, h, 1, -1, out, out, 1, 0, c->tw, s); });
+}
+
+void c2r3d(sf21_hii_ctx* c, float2* in, float* out) {
+  const int n = c->n, h = n / 2 + 1;
+  hipStream_t s = c->stream;
+  launch(c, "cols_half", [&] { launch_cols(n, h, 1, +1, in, in, 1, 0, c->tw, s); });
+  launch(c, "cols_half", [&] { launch_cols(n, h, 0, +1, in, in, 1, 0, c->tw, s); });
+  launch(c, "c2r_rows", [&] { launch_c2r_rows(n, in, out, c->twh, c->tw, s); });
+}
+
+int run_plain(sf21_hii_ctx* c, const float* a, const float* b, const float* cc, double z,
+              float* xhii, double* neutral) {
+  const int n = c->n;
+  const long n3 = c->n3, nh = c->nh;
+  const CellConsts k = cell_consts(c, z);
+  hipStream_t s = c->stream;
+  launch(c, "cell_fields", [&] {
+    launch_cell_fields(n3, a, b, cc, k, c->rec, c->ion, c->fres, c->bubblef, s);
+  });
+  r2c3d(c, c->rec, c->spec2);
+  r2c3d(c, c->ion, c->spec2 + nh);
+  const double dx3 = c->dx * c->dx * c->dx;
+  for (size_t r = 0; r < c->radii.size(); ++r) {
+    const double R = c->radii[r];
+    launch(c, "filter_half",
+           [&] { launch_filter_half(n, c->spec2, c->tmp2, c->wlut + (long)r * c->mlut, s); });
+    c2r3d(c, c->tmp2, c->box_a);
+    c2r3d(c, c->tmp2 + nh, c->box_b);
+    launch(c, "fill_flag", [&] { launch_fill_i32(c->flag_dev, 0, 1, s); });
+    launch(c, "criterion",
+           [&] { launch_criterion(n3, c->box_a, c->box_b, c->bubble, c->flag_dev, s); });
+    if (hipMemcpyAsync(c->flag_host, c->flag_dev, sizeof(int), hipMemcpyDeviceToHost, s) !=
+        hipSuccess)
+      return SF21_ERR_HIP;
+    if (hipStreamSynchronize(s) != hipSuccess) return SF21_ERR_HIP;
+    if (!c->flag_host[0]) continue;
+    if (R >= 2 * c->dx) {
+      launch(c, "ball", [&] { launch_ball(n, c->d2[r], c->box_a, s); });
+      r2c3d(c, c->box_a, c->th);
+      r2c3d(c, c->bubble, c->tmp2);
+      launch(c, "cmul", [&] { launch_cmul(nh, c->tmp2, c->th, s); });
+      c2r3d(c, c->tmp2, c->box_b);
+      launch(c, "threshold", [&] {
+        launch_threshold(n3, c->box_b, (float)(dx3 / (double)n3), dx3 / 100, c->bubblef, s);
+      });
+    } else {
+      const int reach = (int)(R / c->dx) + 1;
+      launch(c, "stencil", [&] { launch_stencil(n, c->bubble, c->d2[r], reach, c->bubblef, s); });
+    }
+  }
+  const int np = clamp_partials_count(n3);
+  launch(c, "clamp_sum", [&] { launch_clamp_sum(n3, c->bubblef, c->fres, c->partials, s); });
+  launch(c, "sum", [&] {
+    launch_sum_partials(c->partials, np, 1.0 / (double)n3, c->neutral_dev, s);
+  });
+  if (hipMemcpyAsync(xhii, c->bubblef, sizeof(float) * n3, hipMemcpyDeviceToDevice, s) !=
+      hipSuccess)
+    return SF21_ERR_HIP;
+  if (hipMemcpyAsync(c->neutral_host, c->neutral_dev, sizeof(double), hipMemcpyDeviceToHost, s) !=
+      hipSuccess)
+    return SF21_ERR_HIP;
+  if (hipStreamSynchronize(s) != hipSuccess) return SF21_ERR_HIP;
+  if (neutral) *neutral = c->neutral_host[0];
+  return check_hip();
+}
+
+void free_all(sf21_hii_ctx* c) {
+  for (auto& g : c->graphs)
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  c->graphs.clear();
+  dev_free(c->tw); dev_free(c->twh); dev_free(c->wlut); dev_free(c->d_d2);
+  dev_free(c->spec); dev_free(c->ubuf); dev_free(c->wmap); dev_free(c->gbuf);
+  dev_free(c->bal_part); dev_free(c->bal); dev_free(c->partials); dev_free(c->neutral_dev);
+  dev_free(c->rec); dev_free(c->ion); dev_free(c->fres); dev_free(c->bubblef);
+  dev_free(c->box_a); dev_free(c->box_b); dev_free(c->bubble);
+  dev_free(c->spec2); dev_free(c->tmp2); dev_free(c->th); dev_free(c->flag_dev);
+  if (c->flag_host) (void)hipHostFree(c->flag_host);
+  if (c->neutral_host) (void)hipHostFree(c->neutral_host);
+  c->flag_host = nullptr;
+  c->neutral_host = nullptr;
+  for (auto e : c->pool) (void)hipEventDestroy(e);
+  c->pool.clear();
+  if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+  c->stream = nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sf21_hii_create(const sf21_hii_config* cfg, sf21_hii_ctx** out) {
+  if (!cfg || !out || !cfg->radii || cfg->n_radii < 1 || !(cfg->box_len > 0)) return SF21_ERR_ARG;
+  if (!size_supported(cfg->n, cfg->path)) return SF21_ERR_ARG;
+  if (cfg->input_mode != SF21_INPUT_SFR && cfg->input_mode != SF21_INPUT_DIRECT) return SF21_ERR_ARG;
+  if (cfg->path != SF21_PATH_FUSED && cfg->path != SF21_PATH_PLAIN) return SF21_ERR_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= cfg->device || cfg->device < 0) {
+    (void)hipGetLastError();
+    return SF21_ERR_HIP;
+  }
+  if (hipSetDevice(cfg->device) != hipSuccess) return SF21_ERR_HIP;
+  auto* c = new sf21_hii_ctx();
+  c->cfg = *cfg;
+  c->radii.assign(cfg->radii, cfg->radii + cfg->n_radii);
+  c->cfg.radii = c->radii.data();
+  c->n = cfg->n;
+  c->n3 = (long)c->n * c->n * c->n;
+  c->nh = (long)c->n * c->n * (c->n / 2 + 1);
+  c->L = cfg->box_len;
+  c->dx = c->L / c->n;
+  c->dk = 2 * kPi / c->L;
+  int rc = SF21_OK;
+  long wmax = -1;
+  for (double R : c->radii) {
+    if (!(R > 0)) { delete c; return SF21_ERR_ARG; }
+    long d = sf21_ball_d2max(R, c->dx);
+    c->d2.push_back((int)d);
+    if (d > wmax) wmax = d;
+  }
+  c->h = (int)std::floor(std::sqrt((double)wmax));
+  while ((long)(c->h + 1) * (c->h + 1) <= wmax) c->h++;
+  while ((long)c->h * c->h > wmax) c->h--;
+  if (c->h > c->n / 2) c->h = c->n / 2;
+  c->K = cfg->radius_batch > 0 ? cfg->radius_batch : 8;
+  if (cfg->filter_tile == 4 || cfg->filter_tile == 8 || cfg->filter_tile == 16)
+    c->tile_filter = cfg->filter_tile;
+  if (c->K > (int)c->radii.size()) c->K = (int)c->radii.size();
+
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return SF21_ERR_HIP;
+  }
+  c->own_stream = true;
+  // tables
+  if ((rc = make_twiddles(c->n, &c->tw)) || (rc = make_twiddles(c->n / 2, &c->twh))) goto fail;
+  {
+    const int hh = c->n / 2;
+    c->mlut = 3L * hh * hh + 1;
+    std::vector<float> w((size_t)c->mlut * c->radii.size());
+    for (size_t r = 0; r < c->radii.size(); ++r)
+      for (long m = 0; m < c->mlut; ++m)
+        w[r * c->mlut + m] = (float)w_filter(c->dk * std::sqrt((double)m) * c->radii[r]);
+    if ((rc = dev_alloc(&c->wlut, w.size()))) goto fail;
+    if (hipMemcpy(c->wlut, w.data(), sizeof(float) * w.size(), hipMemcpyHostToDevice) != hipSuccess) {
+      rc = SF21_ERR_HIP;
+      goto fail;
+    }
+    if ((rc = dev_alloc(&c->d_d2, c->d2.size()))) goto fail;
+    if (hipMemcpy(c->d_d2, c->d2.data(), sizeof(int) * c->d2.size(), hipMemcpyHostToDevice) !=
+        hipSuccess) {
+      rc = SF21_ERR_HIP;
+      goto fail;
+    }
+  }
+  c->neutral_cap = 256;
+  if ((rc = dev_alloc(&c->neutral_dev, (size_t)c->neutral_cap))) goto fail;
+  if (hipHostMalloc(reinterpret_cast<void**>(&c->neutral_host), sizeof(double) * c->neutral_cap) !=
+      hipSuccess) {
+    rc = SF21_ERR_ALLOC;
+    goto fail;
+  }
+  if (hipHostMalloc(reinterpret_cast<void**>(&c->flag_host), sizeof(int)) != hipSuccess) {
+    rc = SF21_ERR_ALLOC;
+    goto fail;
+  }
+  // fused-path buffers (also used by the filtered-field inspection hook)
+  {
+    c->dt.n = c->n;
+    c->dt.h = c->h;
+    c->npart = dt_partials_count(c->n);
+    if ((rc = dev_alloc(&c->spec, (size_t)c->n3)) || (rc = dev_alloc(&c->ubuf, (size_t)c->n3 * c->K)) ||
+        (rc = dev_alloc(&c->wmap, (size_t)c->n3)) || (rc = dev_alloc(&c->gbuf, (size_t)c->n3)) ||
+        (rc = dev_alloc(&c->bal_part, (size_t)2 * balance_partials_count(c->n3))) ||
+        (rc = dev_alloc(&c->bal, 1)) ||
+        (rc = dev_alloc(&c->partials, (size_t)std::max(c->npart, clamp_partials_count(c->n3)))))
+      goto fail;
+  }
+  if (cfg->path == SF21_PATH_PLAIN) {
+    if ((rc = dev_alloc(&c->rec, (size_t)c->n3)) || (rc = dev_alloc(&c->ion, (size_t)c->n3)) ||
+        (rc = dev_alloc(&c->fres, (size_t)c->n3)) || (rc = dev_alloc(&c->bubblef, (size_t)c->n3)) ||
+        (rc = dev_alloc(&c->box_a, (size_t)c->n3)) || (rc = dev_alloc(&c->box_b, (size_t)c->n3)) ||
+        (rc = dev_alloc(&c->bubble, (size_t)c->n3)) || (rc = dev_alloc(&c->spec2, (size_t)(2 * c->nh))) ||
+        (rc = dev_alloc(&c->tmp2, (size_t)(2 * c->nh))) || (rc = dev_alloc(&c->th, (size_t)c->nh)) ||
+        (rc = dev_alloc(&c->flag_dev, 1)))
+      goto fail;
+  }
+  *out = c;
+  return SF21_OK;
+fail:
+  free_all(c);
+  delete c;
+  return rc;
+}
+
+int sf21_hii_destroy(sf21_hii_ctx* c) {
+  if (!c) return SF21_ERR_ARG;
+  (void)hipSetDevice(c->cfg.device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  free_all(c);
+  delete c;
+  return SF21_OK;
+}
+
+int sf21_hii_set_stream(sf21_hii_ctx* c, void* stream) {
+  if (!c) return SF21_ERR_ARG;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto& g : c->graphs)
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  c->graphs.clear();
+  if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+  c->own_stream = false;
+  c->stream = reinterpret_cast<hipStream_t>(stream);
+  return SF21_OK;
+}
+
+int sf21_hii_run_slice_dev(sf21_hii_ctx* c, const float* a, const float* b, const float* cc,
+                           double z, float* xhii, double* neutral) {
+  if (!c || !a || !b || !xhii) return SF21_ERR_ARG;
+  if (hipSetDevice(c->cfg.device) != hipSuccess) return SF21_ERR_HIP;
+  (void)hipGetLastError();
+  int rc;
+  if (c->cfg.path == SF21_PATH_PLAIN) {
+    rc = run_plain(c, a, b, cc, z, xhii, neutral);
+  } else {
+    enqueue_fused(c, a, b, cc, z, xhii, c->neutral_dev, -1, nullptr);
+    if (hipMemcpyAsync(c->neutral_host, c->neutral_dev, sizeof(double), hipMemcpyDeviceToHost,
+                       c->stream) != hipSuccess)
+      return SF21_ERR_HIP;
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return SF21_ERR_HIP;
+    if (neutral) *neutral = c->neutral_host[0];
+    rc = check_hip();
+  }
+  if (c->prof) collect(c);
+  return rc;
+}
+
+int sf21_hii_run_slice(sf21_hii_ctx* c, const float* a, const float* b, const float* cc, double z,
+                       float* xhii, double* neutral) {
+  if (!c || !a || !b || !xhii) return SF21_ERR_ARG;
+  if (hipSetDevice(c->cfg.device) != hipSuccess) return SF21_ERR_HIP;
+  const size_t bytes = sizeof(float) * (size_t)c->n3;
+  float *da = nullptr, *db = nullptr, *dc = nullptr, *dx = nullptr;
+  int rc = SF21_OK;
+  if ((rc = dev_alloc(&da, c->n3)) || (rc = dev_alloc(&db, c->n3)) || (rc = dev_alloc(&dx, c->n3)) ||
+      (cc && (rc = dev_alloc(&dc, c->n3)))) {
+    dev_free(da); dev_free(db); dev_free(dc); dev_free(dx);
+    return rc;
+  }
+  if (hipMemcpy(da, a, bytes, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(db, b, bytes, hipMemcpyHostToDevice) != hipSuccess ||
+      (cc && hipMemcpy(dc, cc, bytes, hipMemcpyHostToDevice) != hipSuccess))
+    rc = SF21_ERR_HIP;
+  if (!rc) rc = sf21_hii_run_slice_dev(c, da, db, dc, z, dx, neutral);
+  if (!rc && hipMemcpy(xhii, dx, bytes, hipMemcpyDeviceToHost) != hipSuccess) rc = SF21_ERR_HIP;
+  dev_free(da); dev_free(db); dev_free(dc); dev_free(dx);
+  return rc;
+}
+
+int sf21_hii_run_batch_dev(sf21_hii_ctx* c, int ns, const float* const* a, const float* const* b,
+                           const float* const* cc, const double* z, float* const* xhii,
+                           double* neutral) {
+  if (!c || ns < 1 || !a || !b || !z || !xhii || ns > c->neutral_cap) return SF21_ERR_ARG;
+  for (int s = 0; s < ns; ++s)
+    if (!a[s] || !b[s] || !xhii[s]) return SF21_ERR_ARG;
+  if (hipSetDevice(c->cfg.device) != hipSuccess) return SF21_ERR_HIP;
+  (void)hipGetLastError();
+  if (c->cfg.path == SF21_PATH_PLAIN) {
+    for (int s = 0; s < ns; ++s) {
+      int rc = run_plain(c, a[s], b[s], cc ? cc[s] : nullptr, z[s], xhii[s], neutral ? &neutral[s] : nullptr);
+      if (rc) return rc;
+    }
+    if (c->prof) collect(c);
+    return SF21_OK;
+  }
+  if (c->prof) {
+    for (int s = 0; s < ns; ++s)
+      enqueue_fused(c, a[s], b[s], cc ? cc[s] : nullptr, z[s], xhii[s], c->neutral_dev + s, -1,
+                    nullptr);
+  } else {
+    // one hipGraph per distinct (pointers, redshifts) tuple, captured once
+    std::vector<const void*> key;
+    std::vector<double> zs(z, z + ns);
+    for (int s = 0; s < ns; ++s) {
+      key.push_back(a[s]);
+      key.push_back(b[s]);
+      key.push_back(cc ? cc[s] : nullptr);
+      key.push_back(xhii[s]);
+    }
+    GraphEntry* ge = nullptr;
+    for (auto& g : c->graphs)
+      if (g.ptrs == key && g.zs == zs) ge = &g;
+    if (!ge) {
+      hipGraph_t graph = nullptr;
+      if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+        return SF21_ERR_HIP;
+      c->capturing = true;
+      for (int s = 0; s < ns; ++s)
+        enqueue_fused(c, a[s], b[s], cc ? cc[s] : nullptr, z[s], xhii[s], c->neutral_dev + s, -1,
+                      nullptr);
+      c->capturing = false;
+      if (hipStreamEndCapture(c->stream, &graph) != hipSuccess) return SF21_ERR_HIP;
+      GraphEntry g;
+      g.ptrs = key;
+      g.zs = zs;
+      hipError_t e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (e != hipSuccess) return SF21_ERR_HIP;
+      if (c->graphs.size() >= 8) {
+        (void)hipGraphExecDestroy(c->graphs.front().exec);
+        c->graphs.erase(c->graphs.begin());
+      }
+      c->graphs.push_back(g);
+      ge = &c->graphs.back();
+    }
+    if (hipGraphLaunch(ge->exec, c->stream) != hipSuccess) return SF21_ERR_HIP;
+  }
+  if (hipMemcpyAsync(c->neutral_host, c->neutral_dev, sizeof(double) * ns, hipMemcpyDeviceToHost,
+                     c->stream) != hipSuccess)
+    return SF21_ERR_HIP;
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return SF21_ERR_HIP;
+  if (neutral)
+    for (int s = 0; s < ns; ++s) neutral[s] = c->neutral_host[s];
+  if (c->prof) collect(c);
+  return check_hip();
+}
+
+int sf21_hii_weight_map(sf21_hii_ctx* c, int* host_out) {
+  if (!c || !host_out) return SF21_ERR_ARG;
+  if (hipSetDevice(c->cfg.device) != hipSuccess) return SF21_ERR_HIP;
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return SF21_ERR_HIP;
+  if (hipMemcpy(host_out, c->wmap, sizeof(int) * c->n3, hipMemcpyDeviceToHost) != hipSuccess)
+    return SF21_ERR_HIP;
+  return SF21_OK;
+}
+
+int sf21_hii_filtered_dev(sf21_hii_ctx* c, const float* a, const float* b, const float* cc,
+                          double z, int radius_index, float* out) {
+  if (!c || !a || !b || !out || radius_index < 0 || radius_index >= (int)c->radii.size())
+    return SF21_ERR_ARG;
+  if (hipSetDevice(c->cfg.device) != hipSuccess) return SF21_ERR_HIP;
+  (void)hipGetLastError();
+  enqueue_fused(c, a, b, cc, z, nullptr, nullptr, radius_index, reinterpret_cast<float2*>(out));
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return SF21_ERR_HIP;
+  if (c->prof) collect(c);
+  return check_hip();
+}
+
+int sf21_hii_profile(sf21_hii_ctx* c, int enable) {
+  if (!c) return SF21_ERR_ARG;
+  c->prof = enable != 0;
+  return SF21_OK;
+}
+
+int sf21_hii_kernel_stats(sf21_hii_ctx* c, int index, char* name, int cap, long* launches,
+                          double* total_ms) {
+  if (!c) return SF21_ERR_ARG;
+  if (index < 0) return (int)c->stats.size();
+  if (index >= (int)c->stats.size()) return SF21_ERR_ARG;
+  if (name && cap > 0) {
+    std::strncpy(name, c->stats[index].name.c_str(), (size_t)cap - 1);
+    name[cap - 1] = 0;
+  }
+  if (launches) *launches = c->stats[index].launches;
+  if (total_ms) *total_ms = c->stats[index].ms;
+  return SF21_OK;
+}
+
+int sf21_hii_reset_stats(sf21_hii_ctx* c) {
+  if (!c) return SF21_ERR_ARG;
+  c->stats.clear();
+  return SF21_OK;
+}
+
+// ---- FFTW-like transforms ------------------------------------------------
+int sf21_fft_plan_3d(int n, int kind, sf21_fft_plan** out) {
+  if (!out || kind < SF21_FFT_R2C || kind > SF21_FFT_C2C_BACKWARD) return SF21_ERR_ARG;
+  const bool real = kind == SF21_FFT_R2C || kind == SF21_FFT_C2R;
+  if (!size_supported(n, real ? SF21_PATH_PLAIN : SF21_PATH_FUSED)) return SF21_ERR_ARG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+    (void)hipGetLastError();
+    return SF21_ERR_HIP;
+  }
+  auto* p = new sf21_fft_plan();
+  p->n = n;
+  p->kind = kind;
+  int rc;
+  if ((rc = make_twiddles(n, &p->tw)) || (rc = make_twiddles(n / 2, &p->twh))) {
+    dev_free(p->tw);
+    dev_free(p->twh);
+    delete p;
+    return rc;
+  }
+  *out = p;
+  return SF21_OK;
+}
+
+int sf21_fft_execute(sf21_fft_plan* p, void* in, void* out, void* stream) {
+  if (!p || !in || !out) return SF21_ERR_ARG;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  const int n = p->n, h = n / 2 + 1;
+  (void)hipGetLastError();
+  switch (p->kind) {
+    case SF21_FFT_R2C: {
+      float2* o = reinterpret_cast<float2*>(out);
+      launch_r2c_rows(n, reinterpret_cast<const float*>(in), o, p->twh, p->tw, s);
+      launch_cols(n, h, 0, -1, o, o, 1, 0, p->tw, s);
+      launch_cols(n, h, 1, -1, o, o, 1, 0, p->tw, s);
+      break;
+    }
+    case SF21_FFT_C2R: {
+      float2* i2 = reinterpret_cast<float2*>(in);
+      launch_cols(n, h, 1, +1, i2, i2, 1, 0, p->tw, s);
+      launch_cols(n, h, 0, +1, i2, i2, 1, 0, p->tw, s);
+      launch_c2r_rows(n, i2, reinterpret_cast<float*>(out), p->twh, p->tw, s);
+      break;
+    }
+    default: {
+      const int sign = p->kind == SF21_FFT_C2C_FORWARD ? -1 : +1;
+      float2* o = reinterpret_cast<float2*>(out);
+      if (in != out &&
+          hipMemcpyAsync(o, in, sizeof(float2) * (size_t)n * n * n, hipMemcpyDeviceToDevice, s) !=
+              hipSuccess)
+        return SF21_ERR_HIP;
+      launch_c2c_rows(n, sign, o, p->tw, s);
+      launch_cols(n, n, 0, sign, o, o, 1, 0, p->tw, s);
+      launch_cols(n, n, 1, sign, o, o, 1, 0, p->tw, s);
+    }
+  }
+  return check_hip();
+}
+
+int sf21_fft_destroy(sf21_fft_plan* p) {
+  if (!p) return SF21_ERR_ARG;
+  dev_free(p->tw);
+  dev_free(p->twh);
+  delete p;
+  return SF21_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,176 @@
+"""Pins of the oracle (the CPU restatement used as the checker) against what
+fixes it independently: numpy's FFT, scipy's spherical Bessel function, the
+photoionisation-equilibrium identity behind XHI, special values of the fits,
+brute-force dilation and lattice-point counts."""
+import ctypes
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+from scipy.special import spherical_jn
+
+import oracle_bind as orc
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = json.load(open(os.path.join(HERE, "golden", "ladders.json")))
+
+
+@pytest.mark.parametrize("n", [8, 12, 16, 30, 32, 48, 64, 96, 128])
+def test_oracle_fft_vs_numpy(n):
+    L = orc.lib()
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal((n, n, n))
+    out = np.zeros((n, n, n // 2 + 1), np.complex128)
+    L.orc_r2c3d_d(n, x.ctypes.data, out.ctypes.data)
+    ref = np.fft.rfftn(x)
+    assert abs(out - ref).max() / abs(ref).max() < 1e-13
+    back = np.zeros((n, n, n))
+    L.orc_c2r3d_d(n, out.ctypes.data, back.ctypes.data)
+    assert abs(back / n ** 3 - x).max() < 1e-12
+    xf = x.astype(np.float32)
+    of = np.zeros((n, n, n // 2 + 1), np.complex64)
+    L.orc_r2c3d_f(n, xf.ctypes.data, of.ctypes.data)
+    assert abs(of - np.fft.rfftn(xf.astype(np.float64))).max() / abs(ref).max() < 2e-6
+
+
+def test_w_filter_vs_bessel():
+    """W(x) = 3 j1(x)/x (auxiliary.c:350-359); the reference switches to the
+    two-term series below x = 0.03, accurate there to O(x^4/280)."""
+    L = orc.lib()
+    xs = np.concatenate([np.linspace(0.03, 60, 2000), [0.031, 1.0, 4.4934, 7.7253]])
+    for x in xs:
+        w = L.orc_W_filter(float(x))
+        ref = 3 * spherical_jn(1, x) / x
+        assert abs(w - ref) < 1e-10 * max(1.0, abs(ref)) + 1e-12, x
+    for x in np.linspace(0, 0.0299, 50):
+        assert abs(L.orc_W_filter(float(x)) - (1 - x * x / 10)) < 1e-15
+        ref = 3 * spherical_jn(1, x) / x if x > 0 else 1.0
+        assert abs(L.orc_W_filter(float(x)) - ref) < 3e-9
+
+
+def test_xhi_equilibrium_identity():
+    """XHI(r) is the neutral fraction x solving x = r (1-x)^2 with x <= 1
+    (user_functions.c:101-106 is the smaller root of r x^2 - (2r+1) x + r)."""
+    L = orc.lib()
+    assert L.orc_XHI(0.0) == 0.0
+    for r in np.geomspace(1e-6, 1e3, 60):
+        x = L.orc_XHI(float(r))
+        assert 0 < x < 1
+        # the reference's expression cancels for small r: up to ~1e-16/r^2 relative
+        tol = 1e-12 + 4e-16 / (r * r)
+        assert abs(x - r * (1 - x) ** 2) <= tol * max(x, 1e-300) + 1e-300, r
+
+
+def test_fit_special_values():
+    """Special values that fix each fitted formula's structure:
+    Rrec(D = r2) = K/16 and Rrec(D -> inf) -> K with K = 1e-24 r1 (1+z)^r4
+    (L_h 3.0857e24/n/h/(1+z))^3 / 1e45; Qion(z) = 1e-45 at a z + b = 45
+    (10^45/V_norm = 1); G_H(0) = 4.19232273531e-13 * 10^12.08875."""
+    L = orc.lib()
+    z, Lh, n, h = 6.0, 100.0, 64, 0.7
+    K = 1e-24 * 9.84815696 * (1 + z) ** 5.06773134 * (Lh * 3.0857e24 / n / h / (1 + z)) ** 3 / 1e45
+    assert math.isclose(L.orc_Rrec(ctypes.c_float(1.76105133).value, z, Lh, n, h), K / 16, rel_tol=1e-6)
+    assert math.isclose(L.orc_Rrec(1e30, z, Lh, n, h), K, rel_tol=1e-6)
+    zq = (45.0 - 5.29944413e+01) / 2.05707274e-02
+    assert math.isclose(L.orc_Qion(zq), 1.0, rel_tol=1e-12)
+    assert math.isclose(L.orc_G_H(0.0), 4.19232273531e-13 * 10 ** 12.08875, rel_tol=1e-12)
+    # Rion's exponential cutoff: Rion(c2) = c1 c2 (1+z)^c4 e^-1 / V_norm
+    c1, c2, c4 = 1.575e39, 9.367608e+07, 2.27044695
+    mf = ctypes.c_float(c2).value
+    ref = c1 * mf * (1 + z) ** c4 * (mf / c2) ** 0.44 * math.exp(-(c2 / mf) ** 3) / 1e45
+    assert math.isclose(L.orc_Rion(mf, z), ref, rel_tol=1e-12)
+
+
+@pytest.mark.parametrize("case", GOLD["cases"], ids=lambda c: c["name"])
+def test_ladder_reference_formula(case):
+    dx = case["side"] / case["n"]
+    nb = round(math.log(case["rmax"] / dx) / math.log(1 / 0.9))
+    assert nb == case["nbins"]
+    radii, nm = orc.ladder(case["rmax"], dx, nb)
+    assert nm == case["n_main"] and len(radii) == case["n_total"]
+    bf = 10 ** (math.log10(case["rmax"] / dx) / nb)
+    assert abs(bf - case["bfactor"]) < 5e-7
+    assert all(r >= 2 * dx for r in radii[:nm]) and all(dx <= r < 2 * dx for r in radii[nm:])
+
+
+def test_d2max_bruteforce():
+    for dx in [0.78125, 1.171875, 1.5625, 0.3]:
+        for R in np.linspace(dx, 40 * dx, 157):
+            m = orc.d2max(float(R), dx)
+            assert math.sqrt(m) * dx <= R and not (math.sqrt(m + 1) * dx <= R)
+
+
+def test_lattice_ball_counts():
+    """One centre of weight m paints exactly the lattice points |d|^2 <= m."""
+    lb = GOLD["lattice_ball_counts"]
+    n = 16
+    for m, cnt in zip(lb["m"], lb["count"]):
+        w = np.full(n ** 3, -1, np.int32)
+        w[(5 * n + 7) * n + 9] = m
+        assert orc.dilate_brute(n, w).sum() == cnt
+
+
+def test_oracle_uniform_known_answers():
+    """Uniform fields: every filtered field equals the input (W(0) = 1, no
+    other mode).  ion >= rec everywhere ionises the box; ion = c*rec with
+    c < 1 leaves the quick-fill value c in every cell."""
+    n, side = 16, 32.0
+    radii = [8.0, 5.0, 3.0, 2.5]
+    rec = np.full(n ** 3, 2.0, np.float32)
+    for prec in (32, 64):
+        for dil in (0, 1):
+            x, neu, fc = orc.hii_slice(n, side, radii, rec, np.full(n ** 3, 2.5, np.float32), None, prec, dil)
+            assert np.all(x == 1.0) and neu == 0.0 and np.all(fc == 0)
+            x, neu, fc = orc.hii_slice(n, side, radii, rec, np.full(n ** 3, 0.5, np.float32), None, prec, dil)
+            assert np.allclose(x, 0.25) and abs(neu - 0.75) < 1e-7 and np.all(fc == -1)
+
+
+def test_oracle_clamp_by_fresid():
+    n, side = 16, 32.0
+    rec = np.full(n ** 3, 1.0, np.float32)
+    fr = np.full(n ** 3, 0.125, np.float32)
+    x, neu, _ = orc.hii_slice(n, side, [8.0, 3.0], rec, np.full(n ** 3, 4.0, np.float32), fr, 64, 1)
+    assert np.all(x == np.float32(0.875)) and abs(neu - 0.125) < 1e-12
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_exact_dilation_equals_fp64_reference_dataflow(seed):
+    """The FP64 run of the reference's dataflow (ball FFT convolution with
+    count > 0.01, stencil below 2dx) equals the exact ball dilation, and both
+    equal brute-force dilation of the first-centre weights."""
+    from helpers import direct_case, weights_from_first_centre
+    case = direct_case(32, 1.0, seed=seed)
+    n, side, radii = case["n"], case["side"], case["radii"]
+    x0, neu0, fc0 = orc.hii_slice(n, side, radii, case["rec"], case["ion"], None, 64, 0)
+    x1, neu1, fc1 = orc.hii_slice(n, side, radii, case["rec"], case["ion"], None, 64, 1)
+    assert np.array_equal(fc0, fc1)
+    assert np.array_equal(x0, x1) and neu0 == neu1
+    w = weights_from_first_centre(fc1, radii, side / n)
+    dil = orc.dilate_brute(n, w).astype(bool)
+    rec, ion = case["rec"].ravel(), case["ion"].ravel()
+    with np.errstate(divide="ignore", invalid="ignore"):
+        q = np.where(rec > 0, np.minimum(ion.astype(np.float64) / rec, 1.0), 1.0)
+    qf = np.where(ion > 0, q, 0.0).astype(np.float32)
+    assert np.array_equal(x1, np.where(dil, np.float32(1), qf))
+
+
+def test_filtered_vs_numpy_independent():
+    """The oracle's filter (get_HIIbubbles.c:305-325) against numpy's FFT with
+    numpy.fft.fftfreq frequencies and scipy's spherical Bessel window."""
+    from helpers import direct_case
+    case = direct_case(32, 1.0)
+    n, side = case["n"], case["side"]
+    R = case["radii"][10]
+    rr, ii = orc.filtered(n, side, R, case["rec"], case["ion"], 64)
+    k1 = np.fft.fftfreq(n, d=side / n) * 2 * np.pi
+    kz = np.fft.rfftfreq(n, d=side / n) * 2 * np.pi
+    kk = np.sqrt(k1[:, None, None] ** 2 + k1[None, :, None] ** 2 + kz[None, None, :] ** 2)
+    x = kk * R
+    with np.errstate(divide="ignore", invalid="ignore"):
+        W = np.where(x > 0, 3 * spherical_jn(1, x) / np.where(x > 0, x, 1), 1.0)
+    for box, got in ((case["rec"], rr), (case["ion"], ii)):
+        ref = np.fft.irfftn(np.fft.rfftn(box.reshape(n, n, n).astype(np.float64)) * W,
+                            s=(n, n, n), axes=(0, 1, 2)) * n ** 3
+        assert abs(got.reshape(n, n, n) - ref).max() / abs(ref).max() < 1e-9
