@@ -110,6 +110,14 @@ struct Dft<5, SIGN> {
   }
 };
 
+// Compiler-level fence for the LDS exchanges.  __syncthreads() orders the
+// hardware, but the compiler was observed to sink a stage's LDS reads past
+// the following barrier into a later basic block (k_filter_i: the forward
+// transform's last reads landed after the barrier, racing with the inverse
+// transform's first LDS writes of another wave).  An empty asm with a memory
+// clobber pins every LDS access of an exchange on its side of the barrier.
+__device__ __forceinline__ void lds_pin() { asm volatile("" ::: "memory"); }
+
 // LDS index padding: one slot every 8 breaks the stride-R write pattern of
 // the first stage.
 __device__ __forceinline__ int lds_pad(int i) { return i + (i >> 3); }
@@ -182,12 +190,14 @@ __device__ __forceinline__ void stages(float2 (&v)[E], float* sre, float* sim, i
     }
   }
   if constexpr (!LAST) {
+    lds_pin();
     __syncthreads();
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       const int o = lds_pad(t + TPL * m);
       v[m] = make_float2(sre[o], sim[o]);
     }
+    lds_pin();
     __syncthreads();
     stages<L, E, SIGN, NS * R, REST...>(v, sre, sim, t, tw);
   }

@@ -443,6 +443,7 @@ __global__ __launch_bounds__(256) void k_dt_tile(DtArgs a, int axis, const int* 
     const int e = t / kDtCols, c = t % kDtCols;
     sm[c * cs + e + (e >> 3)] = in[base + e * stride + c];
   }
+  lds_pin();
   __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   for (int c = wave; c < kDtCols; c += blockDim.x >> 6) {
@@ -455,6 +456,7 @@ __global__ __launch_bounds__(256) void k_dt_tile(DtArgs a, int axis, const int* 
         col[e + (e >> 3)] = o[q];
       }
   }
+  lds_pin();
   __syncthreads();
   for (int t = threadIdx.x; t < n * kDtCols; t += blockDim.x) {
     const int e = t / kDtCols, c = t % kDtCols;
@@ -485,6 +487,7 @@ __global__ __launch_bounds__(256) void k_dt_rows_final(DtArgs a, const int* in,
       const int* src = in + row * n;
       for (int e = lane; e < n; e += 64) col[e + (e >> 3)] = src[e];
     }
+    lds_pin();
     __syncthreads();
     int o[kQmax];
     if (valid) dt_line_wave<kQmax>(col, n, Q, a.h, lane, o);
@@ -498,6 +501,7 @@ __global__ __launch_bounds__(256) void k_dt_rows_final(DtArgs a, const int* in,
         xhii[idx] = v;
         acc += 1.0 - (double)v;
       }
+    lds_pin();
     __syncthreads();
   }
   __shared__ double red[256];

@@ -60,6 +60,7 @@ __global__ __launch_bounds__(256) void k_r2c_rows(const float* __restrict__ in,
     sre[o] = v[m].x;
     sim[o] = v[m].y;
   }
+  lds_pin();
   __syncthreads();
   float2* __restrict__ rout = out + row * (H + 1);
 #pragma unroll

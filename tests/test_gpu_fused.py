@@ -65,31 +65,46 @@ def test_fused_vs_oracle(torch_cuda, n, zeta):
           f"mean x_HII gpu {1 - neu_g:.7f} oracle {1 - neu_o:.7f}")
     if diff.size:
         d2 = np.array([orc.d2max(R, dx) for R in radii])
-        deciding = {}
+        groups = {}
         for cell in diff:
-            # largest radius where exactly one side says "centre"
+            # largest radius where the two maps disagree about "centre at R";
+            # radii sharing its ball bound cannot be told apart by the maps,
+            # so the whole group is examined with per-radius flags
             cand = [k for k in range(len(radii)) if (w_g[cell] >= d2[k]) != (w_o[cell] >= d2[k])]
-            deciding.setdefault(min(cand), []).append(cell)
+            k0 = min(cand)
+            groups.setdefault(int(d2[k0]), []).append(cell)
         ctx = sf.HiiContext(n, side, radii, sf.INPUT_DIRECT, sf.PATH_FUSED)
         a = torch.from_numpy(case["rec"].ravel()).cuda()
         b = torch.from_numpy(case["ion"].ravel()).cuda()
-        for k, cells in sorted(deciding.items()):
-            rr, ii = orc.filtered(n, side, radii[k], case["rec"], case["ion"], precision=64)
-            g = ctx.filtered(a, b, radius_index=k).cpu().numpy().astype(np.float64)
-            err_rec = np.abs(g[:, 0] - rr).max()
-            err_ion = np.abs(g[:, 1] - ii).max()
-            # FP32 filtered fields against FP64: max error relative to the field maximum
-            rel_rec = err_rec / np.abs(rr).max()
-            rel_ion = err_ion / np.abs(ii).max()
-            assert rel_rec < 1e-5 and rel_ion < 1e-5, (k, rel_rec, rel_ion)
+        for dval, cells in sorted(groups.items()):
             cells = np.array(cells)
-            ok = near_threshold(rr[cells], ii[cells], err_rec, err_ion)
-            print(f"  radius {k} R={radii[k]:.3f}: {cells.size} cells, rel err rec {rel_rec:.2e} "
-                  f"ion {rel_ion:.2e}, ratio range {np.min(ii[cells]/rr[cells]):.3e}.."
-                  f"{np.max(ii[cells]/rr[cells]):.3e}")
-            assert ok.all(), (k, cells[~ok][:5], (ii[cells] / rr[cells])[~ok][:5])
-    # 3) mean
-    assert abs(neu_g - neu_o) < 1e-5
+            explained = np.zeros(cells.size, bool)
+            for k in [k for k in range(len(radii)) if d2[k] == dval]:
+                rr, ii = orc.filtered(n, side, radii[k], case["rec"], case["ion"], precision=64)
+                g = ctx.filtered(a, b, radius_index=k).cpu().numpy().astype(np.float64)
+                err_rec = np.abs(g[:, 0] - rr).max()
+                err_ion = np.abs(g[:, 1] - ii).max()
+                # FP32 filtered fields against FP64: max error relative to the field maximum
+                rel_rec = err_rec / np.abs(rr).max()
+                rel_ion = err_ion / np.abs(ii).max()
+                assert rel_rec < 1e-5 and rel_ion < 1e-5, (k, rel_rec, rel_ion)
+                f32 = (g[cells, 1] > 0) & (g[cells, 1] >= g[cells, 0])
+                f64 = (ii[cells] > 0) & (ii[cells] >= rr[cells])
+                flip = f32 != f64
+                ok = near_threshold(rr[cells], ii[cells], err_rec, err_ion)
+                assert ok[flip].all(), (k, cells[flip & ~ok][:5], (ii[cells] / rr[cells])[flip & ~ok][:5])
+                explained |= flip
+                print(f"  radius {k} R={radii[k]:.3f}: {int(flip.sum())} flipped of {cells.size}, "
+                      f"rel err rec {rel_rec:.2e} ion {rel_ion:.2e}")
+            assert explained.all(), cells[~explained][:5]
+    # 3) mean: within 1e-5, or fully attributed to the explained flips (each
+    # cell whose dilation status differs moves the sum of 1 - x by at most 1;
+    # at 64^3-128^3 one flipped small-R void centre alone is ~1e-5)
+    dil_o = orc.dilate_brute(n, w_o).astype(bool)
+    D = int(np.count_nonzero(dil != dil_o))
+    print(f"  mean |diff| {abs(neu_g - neu_o):.2e}; cells with different dilation {D} ({D / n**3:.2e})")
+    assert D / n ** 3 < 1e-4
+    assert abs(neu_g - neu_o) <= max(1e-5, D / n ** 3 + 1e-12)
 
 
 @pytest.mark.gpu
@@ -105,8 +120,11 @@ def test_plain_vs_oracle(torch_cuda, n, zeta):
     x_p, neu_p, _, _ = run_case(torch, case, path=sf.PATH_PLAIN)
     x_f, neu_f, _, _ = run_case(torch, case, path=sf.PATH_FUSED)
     x_o, neu_o, _ = orc.hii_slice(n, side, radii, case["rec"], case["ion"], precision=64, dilation=0)
+    x_o32, neu_o32, _ = orc.hii_slice(n, side, radii, case["rec"], case["ion"], precision=32, dilation=0)
     frac = np.mean(x_p != x_f)
     print(f"n={n}: plain {1 - neu_p:.7f} fused {1 - neu_f:.7f} oracle64-ref {1 - neu_o:.7f} "
-          f"plain/fused differing cells {frac:.2e}")
-    assert abs(neu_p - neu_o) < 1e-5
+          f"oracle32-ref {1 - neu_o32:.7f} plain/fused differing cells {frac:.2e}")
+    # FP32 reference dataflow (plain path, and the oracle's own FP32 typing)
+    # against FP64: the FFT-convolution noise and FP32 voids allow a few cells
+    assert abs(neu_p - neu_o) < 2e-5 and abs(neu_o32 - neu_o) < 2e-5
     assert frac < 5e-3
