@@ -39,6 +39,9 @@ extern "C" {
                                 use_SFR = T path, get_HIIbubbles.c:216-225, :262-275 */
 #define SF21_INPUT_DIRECT 1  /* a = recombination box, b = emissivity box, c = residual
                                 neutral fraction box (nullable -> 0) */
+#define SF21_INPUT_HALO 2    /* a = delta, b = emissivity box deposited from a halo catalog
+                                (sf21_hii_halo_emissivity_dev): the reference's
+                                use_SFR = F path, get_HIIbubbles.c:216-259 */
 
 /* Execution paths. */
 #define SF21_PATH_FUSED 0    /* fused exact finder: filtered fields of several radii per
@@ -63,6 +66,9 @@ typedef struct sf21_hii_config {
   int radius_batch;      /* radii per fused filter pass, 0 = automatic */
   int device;            /* HIP device ordinal */
   int filter_tile;       /* columns per block of the fused filter pass: 4, 8, 16 or 0 (auto) */
+  int filter_w;          /* top-hat weights: 0 = FP64 table per integer |k|^2 rounded to float,
+                            1 = FP32 evaluation in the kernel */
+  int n_halo;            /* N_halo, halo mode: the catalog grid (>= n) */
 } sf21_hii_config;
 
 const char *sf21_strerror(int code);
@@ -88,6 +94,16 @@ int sf21_hii_run_slice_dev(sf21_hii_ctx *ctx, const float *a, const float *b,
 int sf21_hii_run_batch_dev(sf21_hii_ctx *ctx, int n_slices, const float *const *a,
                            const float *const *b, const float *const *c, const double *z,
                            float *const *xhii, double *neutral);
+
+/* Halo mode.  halos: n_halos records {float Mass, x, y, z} with positions
+ * in N_halo cell units (auxiliary.h:13-17, output_files.txt:10-14).
+ * Emissivity box = smooth_box(CIC(Rion(M, z))) * (dx_s/dx_h)^3 * fesc
+ * (get_HIIbubbles.c:233-259); _dev takes device pointers.  run_slice_halo
+ * takes host delta and catalog and runs the whole slice. */
+int sf21_hii_halo_emissivity_dev(sf21_hii_ctx *ctx, const float *halos, long n_halos, double z,
+                                 float *ion_out);
+int sf21_hii_run_slice_halo(sf21_hii_ctx *ctx, const float *delta, const float *halos,
+                            long n_halos, double z, float *xhii, double *neutral);
 
 /* Inspection for tests (fused path): per-cell ball bound d2max(R) of the
  * largest radius at which the cell was a bubble centre in the last slice, -1

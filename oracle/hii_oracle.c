@@ -130,17 +130,21 @@ void orc_prepass_sfr(long n3, const float *delta, const float *sfrd, double z,
 }
 
 /* auxiliary.c:761-808: cloud-in-cell deposit of v at (x1,y1,z1) (cell
- * units) with periodic wrap.  For non-negative positions the reference's
- * sign handling is inert and its weights are trilinear; they are formed the
- * way the reference rounds them: the "far" fraction is first stored back
- * into a float (1 - frac), and each factor 1 - w, their product and the
- * accumulation are FP64, rounded once into the float map. */
+ * units) with periodic wrap.  Per axis: truncate toward zero; a negative
+ * remainder steps one cell down and its magnitude becomes the far weight
+ * (:764-784).  Weights are formed the way the reference rounds them: the
+ * far fraction is first stored back into a float (1 - frac), and each
+ * factor 1 - w, their product and the accumulation are FP64, rounded once
+ * into the float map. */
 void orc_cic(float x1, float y1, float z1, float v, float *map, long n) {
   long x = (long)x1, y = (long)y1, z = (long)z1;
   float f[3];
   double w[3][2];
   int a, b, c, d;
   f[0] = x1 - (float)x; f[1] = y1 - (float)y; f[2] = z1 - (float)z;
+  if (f[0] < 0.0f) { x -= 1; f[0] = fabsf(f[0]); }
+  if (f[1] < 0.0f) { y -= 1; f[1] = fabsf(f[1]); }
+  if (f[2] < 0.0f) { z -= 1; f[2] = fabsf(f[2]); }
   for (d = 0; d < 3; d++) {
     float far = (float)(1.0 - f[d]);
     w[d][0] = 1.0 - f[d];

@@ -32,6 +32,7 @@ SF21_ERR_FORMAT = -5
 
 INPUT_SFR = 0
 INPUT_DIRECT = 1
+INPUT_HALO = 2
 PATH_FUSED = 0
 PATH_PLAIN = 1
 
@@ -57,6 +58,8 @@ class HiiConfig(ctypes.Structure):
         ("radius_batch", ctypes.c_int),
         ("device", ctypes.c_int),
         ("filter_tile", ctypes.c_int),
+        ("filter_w", ctypes.c_int),
+        ("n_halo", ctypes.c_int),
     ]
 
 
@@ -111,6 +114,8 @@ def _lib():
     L.sf21_hii_profile.argtypes = [vp, i]
     L.sf21_hii_kernel_stats.argtypes = [vp, i, ctypes.c_char_p, i, ctypes.POINTER(ctypes.c_long), dp]
     L.sf21_hii_reset_stats.argtypes = [vp]
+    L.sf21_hii_halo_emissivity_dev.argtypes = [vp, vp, ctypes.c_long, d, vp]
+    L.sf21_hii_run_slice_halo.argtypes = [vp, vp, vp, ctypes.c_long, d, vp, dp]
     L.sf21_fft_plan_3d.argtypes = [i, i, ctypes.POINTER(vp)]
     L.sf21_fft_execute.argtypes = [vp, vp, vp, vp]
     L.sf21_fft_destroy.argtypes = [vp]
@@ -179,12 +184,13 @@ class HiiContext:
 
     def __init__(self, n: int, box_len: float, radii, input_mode: int = INPUT_DIRECT,
                  path: int = PATH_FUSED, hubble: float = 0.7, fesc: float = 1.0,
-                 radius_batch: int = 0, device: int = 0, filter_tile: int = 0):
+                 radius_batch: int = 0, device: int = 0, filter_tile: int = 0, filter_w: int = 0,
+                 n_halo: int = 0):
         self.n = int(n)
         self.radii = [float(r) for r in radii]
         self._radii = (ctypes.c_double * len(self.radii))(*self.radii)
         cfg = HiiConfig(self.n, float(box_len), len(self.radii), self._radii, input_mode, path,
-                        float(hubble), float(fesc), int(radius_batch), int(device), int(filter_tile))
+                        float(hubble), float(fesc), int(radius_batch), int(device), int(filter_tile), int(filter_w), int(n_halo))
         self._h = ctypes.c_void_p()
         _check(_lib().sf21_hii_create(ctypes.byref(cfg), ctypes.byref(self._h)), "sf21_hii_create")
         self.path = path
@@ -242,6 +248,16 @@ class HiiContext:
         _check(_lib().sf21_hii_run_batch_dev(self._h, ns, pa, pb, pc, zs, po, neu),
                "sf21_hii_run_batch_dev")
         return out_list, list(neu)
+
+    def halo_emissivity(self, halos, z: float, out=None):
+        """Device catalog tensor (n, 4) float32 -> emissivity box (device)."""
+        import torch
+        if out is None:
+            out = torch.empty(self.n ** 3, dtype=torch.float32, device=halos.device)
+        torch.cuda.current_stream().synchronize()
+        _check(_lib().sf21_hii_halo_emissivity_dev(self._h, _ptr(halos), int(halos.shape[0]), float(z),
+                                                   _ptr(out)), "sf21_hii_halo_emissivity_dev")
+        return out
 
     def weight_map(self) -> np.ndarray:
         out = np.empty(self.n ** 3, np.int32)

@@ -29,7 +29,8 @@ void launch_fwd_rows(int n, const float* a, const float* b, const float* c, Cell
 void launch_cols(int n, int width, int axis, int sign, const float2* in, float2* out,
                  int nbatch, long bstride, const float2* tw, hipStream_t s);
 void launch_filter_i(int n, const float2* spec, float2* u, int nrad, long ustride,
-                     const float* wlut, long m_lut, const float2* tw, hipStream_t s, int tile);
+                     const float* wlut, long m_lut, const float* kappa, const float2* tw,
+                     hipStream_t s, int tile, int wmode);
 void launch_inv_rows_crit(int n, const float2* v, int nrad, long vstride, const int* d2max,
                           const float* bal, int* wmap, float2* dbg, hipStream_t s,
                           const float2* tw);
@@ -62,5 +63,10 @@ void launch_stencil(int n, const float* centres, int d2, int reach, float* bubbl
 int clamp_partials_count(long n3);
 void launch_clamp_sum(long n3, float* bubblef, const float* fresid, double* partials,
                       hipStream_t s);
+
+// ---- halo-catalog emissivity ----
+void launch_halo_emissivity(long nhalos, const float* halos, int ng, int ns, int sm, float k_rion,
+                            float c2, float c3, float scale, float* grid, float* out,
+                            hipStream_t s);
 
 }  // namespace sf21

@@ -215,50 +215,77 @@ __global__ __launch_bounds__(TILE*(L / Plan<L>::E)) void k_cols(const float2* in
 // an FP64 table indexed by the integer |k|^2, rounded to float), inverse
 // along i.  The forward spectrum along i never leaves the registers.
 // ------------------------------------------------------------------------
-template <int L, int TILE>
+// W(x) = 3 j1(x)/x in FP32 (auxiliary.c:350-359 evaluated on the fly): the
+// Taylor series 3 sum_k (-1)^(k+1) 2k x^(2k-2)/(2k+1)! to x^12 below 1.25
+// (truncation < 1e-11), 3 (sin x - x cos x)/x^3 above, where the
+// subtraction no longer cancels.
+__device__ __forceinline__ float w_tophat(float x) {
+  if (x < 1.25f) {
+    const float q = x * x;
+    return 1.0f + q * (-0.1f + q * (1.0f / 280.0f + q * (-1.0f / 15120.0f +
+           q * (1.0f / 1330560.0f + q * (-1.0f / 172972800.0f + q * (1.0f / 31135104000.0f))))));
+  }
+  float sn, cs;
+  sincosf(x, &sn, &cs);
+  return 3.0f * (sn - x * cs) / (x * x * x);
+}
+
+template <int L, int TILE, bool WTAB>
 __global__ __launch_bounds__(TILE*(L / Plan<L>::E)) void k_filter_i(
     const float2* __restrict__ spec, float2* __restrict__ u, int nrad, long ustride,
-    const float* __restrict__ wlut, long mlut, const float2* __restrict__ tw) {
+    const float* __restrict__ wlut, long mlut, const float* __restrict__ kappa,
+    const float2* __restrict__ tw) {
   constexpr int E = Plan<L>::E, TPL = L / E, NT = L / TILE;
   __shared__ float lds[TILE * 2 * LdsLine<L>::kFloats];
   const int c = threadIdx.x % TILE, t = threadIdx.x / TILE;
   const int kj = blockIdx.x / NT;
   const int kp = (blockIdx.x % NT) * TILE + c;
-  const long base = (long)kj * L + kp, stride = (long)L * L;
+  // element e of this line sits at line[e * L * L] (32-bit offsets: L^3 < 2^31)
+  const float2* __restrict__ src = spec + (long)kj * L + kp;
   float* sre = lds + c * 2 * LdsLine<L>::kFloats;
   float* sim = sre + LdsLine<L>::kFloats;
   float2 z[E];
 #pragma unroll
-  for (int m = 0; m < E; ++m) z[m] = spec[base + (long)(t + TPL * m) * stride];
+  for (int m = 0; m < E; ++m) z[m] = src[(t + TPL * m) * (L * L)];
   line_fft<L, -1>(z, sre, sim, t, tw);
   const int mjp = fold_sq(kj, L) + fold_sq(kp, L);
-  int ms[E];
-#pragma unroll
-  for (int m = 0; m < E; ++m) ms[m] = mjp + fold_sq(t + TPL * m, L);
 #pragma unroll 1
   for (int r = 0; r < nrad; ++r) {
-    const float* __restrict__ w = wlut + (long)r * mlut;
     float2 v[E];
+    if constexpr (WTAB) {
+      const float* __restrict__ w = wlut + (long)r * mlut;
 #pragma unroll
-    for (int m = 0; m < E; ++m) {
-      const float wv = w[ms[m]];
-      v[m] = make_float2(z[m].x * wv, z[m].y * wv);
+      for (int m = 0; m < E; ++m) {
+        const float wv = w[mjp + fold_sq(t + TPL * m, L)];
+        v[m] = make_float2(z[m].x * wv, z[m].y * wv);
+      }
+    } else {
+      const float kr = kappa[r];  // dk * R
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const float wv = w_tophat(kr * sqrtf((float)(mjp + fold_sq(t + TPL * m, L))));
+        v[m] = make_float2(z[m].x * wv, z[m].y * wv);
+      }
     }
     line_fft<L, +1>(v, sre, sim, t, tw);
-    float2* __restrict__ ur = u + (long)r * ustride;
+    float2* __restrict__ dst = u + (long)r * ustride + (long)kj * L + kp;
 #pragma unroll
-    for (int m = 0; m < E; ++m) ur[base + (long)(t + TPL * m) * stride] = v[m];
+    for (int m = 0; m < E; ++m) dst[(t + TPL * m) * (L * L)] = v[m];
   }
 }
 
 // ------------------------------------------------------------------------
-// Per radius: inverse along p -> (N^3 rec_R, N^3 ion_R), centre test, and
-// wmap[cell] = max(wmap[cell], d2max[r]) for centres.  blockIdx.y = radius
-// within the batch.  dbg (nullable) receives the filtered pair.
+// Per radius batch: inverse along p -> (N^3 rec_R, s N^3 ion_R) for each of
+// the nrad radii, centre test, and for every cell the largest ball bound
+// among the batch's radii at which it is a centre.  Batches are issued in
+// ascending radius, so a batch's bounds exceed every earlier batch's and a
+// plain store (only where some centre was found) leaves in wmap the bound of
+// the largest radius -- no atomics and no reads of the map.  dbg (nullable)
+// receives the unscaled filtered pairs.
 // ------------------------------------------------------------------------
 template <int L>
 __global__ __launch_bounds__(256) void k_inv_rows_crit(const float2* __restrict__ vin,
-                                                       long vstride,
+                                                       int nrad, long vstride,
                                                        const int* __restrict__ d2max,
                                                        const float* __restrict__ bal,
                                                        int* __restrict__ wmap,
@@ -268,23 +295,30 @@ __global__ __launch_bounds__(256) void k_inv_rows_crit(const float2* __restrict_
   __shared__ float lds[LPB * 2 * LdsLine<L>::kFloats];
   const int lw = threadIdx.x / TPL, t = threadIdx.x % TPL;
   const long row = (long)blockIdx.x * LPB + lw;
-  const int r = blockIdx.y;
-  const float2* __restrict__ src = vin + (long)r * vstride;
   float* sre = lds + lw * 2 * LdsLine<L>::kFloats;
   float* sim = sre + LdsLine<L>::kFloats;
-  float2 v[E];
-#pragma unroll
-  for (int m = 0; m < E; ++m) v[m] = src[row * L + t + TPL * m];
-  line_fft<L, +1>(v, sre, sim, t, tw);
-  const int w = d2max[r];
   const float sc = bal[0];
+  int wbest[E];
 #pragma unroll
-  for (int m = 0; m < E; ++m) {
-    const long idx = row * L + t + TPL * m;
-    // Re = N^3 rec_R, Im = s N^3 ion_R: centre test on (s rec_R, s ion_R)
-    if (is_centre(v[m].x * sc, v[m].y)) atomicMax(&wmap[idx], w);
-    if (dbg) dbg[(long)r * vstride + idx] = make_float2(v[m].x, v[m].y / sc);
+  for (int m = 0; m < E; ++m) wbest[m] = -1;
+#pragma unroll 1
+  for (int r = 0; r < nrad; ++r) {
+    const float2* __restrict__ src = vin + (long)r * vstride + row * L;
+    float2 v[E];
+#pragma unroll
+    for (int m = 0; m < E; ++m) v[m] = src[t + TPL * m];
+    line_fft<L, +1>(v, sre, sim, t, tw);
+    const int w = d2max[r];
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      // Re = N^3 rec_R, Im = s N^3 ion_R: centre test on (s rec_R, s ion_R)
+      if (is_centre(v[m].x * sc, v[m].y)) wbest[m] = max(wbest[m], w);
+      if (dbg) dbg[(long)r * vstride + row * L + t + TPL * m] = make_float2(v[m].x, v[m].y / sc);
+    }
   }
+#pragma unroll
+  for (int m = 0; m < E; ++m)
+    if (wbest[m] >= 0) wmap[row * L + t + TPL * m] = wbest[m];
 }
 
 __global__ void k_fill_i32(int* __restrict__ p, int value, long count) {
@@ -577,12 +611,17 @@ void launch_cols(int n, int width, int axis, int sign, const float2* in, float2*
 }
 
 void launch_filter_i(int n, const float2* spec, float2* u, int nrad, long ustride,
-                     const float* wlut, long m_lut, const float2* tw, hipStream_t s, int tile) {
+                     const float* wlut, long m_lut, const float* kappa, const float2* tw,
+                     hipStream_t s, int tile, int wmode) {
 #define X(L, TL)                                                                          \
   if (n == L && tile == TL) {                                                             \
     constexpr int NT = TL * (L / Plan<L>::E);                                             \
-    hipLaunchKernelGGL((k_filter_i<L, TL>), dim3(L * (L / TL)), dim3(NT), 0, s, spec, u, nrad, \
-                       ustride, wlut, m_lut, tw);                                         \
+    if (wmode == 0)                                                                       \
+      hipLaunchKernelGGL((k_filter_i<L, TL, true>), dim3(L * (L / TL)), dim3(NT), 0, s, spec, u, \
+                         nrad, ustride, wlut, m_lut, kappa, tw);                          \
+    else                                                                                  \
+      hipLaunchKernelGGL((k_filter_i<L, TL, false>), dim3(L * (L / TL)), dim3(NT), 0, s, spec, u, \
+                         nrad, ustride, wlut, m_lut, kappa, tw);                          \
     return;                                                                               \
   }
 #define Y(L) X(L, 4) X(L, 8) X(L, 16)
@@ -597,7 +636,7 @@ void launch_inv_rows_crit(int n, const float2* v, int nrad, long vstride, const 
 #define X(L)                                                                              \
   if (n == L) {                                                                           \
     constexpr int LPB = 256 / (L / Plan<L>::E);                                           \
-    hipLaunchKernelGGL(k_inv_rows_crit<L>, dim3((L * L) / LPB, nrad), dim3(256), 0, s, v, vstride, \
+    hipLaunchKernelGGL(k_inv_rows_crit<L>, dim3((L * L) / LPB), dim3(256), 0, s, v, nrad, vstride, \
                        d2max, bal, wmap, dbg, tw);                                        \
     return;                                                                               \
   }

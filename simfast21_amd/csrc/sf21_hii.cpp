@@ -74,7 +74,9 @@ struct sf21_hii_ctx {
   std::vector<double> radii;
   std::vector<int> d2;
   int n = 0, h = 0, K = 0;
-  int tile_filter = 8;
+  int tile_filter = 16;
+  int filter_w = 0;  // 0: FP64 table per |k|^2, 1: FP32 on the fly
+  float* d_kappa = nullptr;
   long n3 = 0, nh = 0;
   double L = 0, dx = 0, dk = 0;
   hipStream_t stream = nullptr;
@@ -101,6 +103,10 @@ struct sf21_hii_ctx {
   float2 *spec2 = nullptr, *tmp2 = nullptr, *th = nullptr;
   int* flag_dev = nullptr;
   int* flag_host = nullptr;
+  // halo mode
+  float* halo_grid = nullptr;   // N_halo^3 deposit grid
+  float* halo_buf = nullptr;    // catalog upload (host entry point)
+  long halo_cap = 0;
   // profiling
   bool prof = false;
   std::vector<KStat> stats;
@@ -152,7 +158,9 @@ int make_twiddles(int len, float2** out) {
 // :273 with user_functions.c:58-65, :84-97), FP64 on the host.
 CellConsts cell_consts(const sf21_hii_ctx* c, double z) {
   CellConsts k{};
-  k.mode = c->cfg.input_mode == SF21_INPUT_SFR ? 0 : 1;
+  // halo mode shares the SFR pre-pass (rec, fresid from delta) with the
+  // emissivity box taken as given (k_ion = 1)
+  k.mode = (c->cfg.input_mode == SF21_INPUT_SFR || c->cfg.input_mode == SF21_INPUT_HALO) ? 0 : 1;
   if (k.mode == 0) {
     const double h = c->cfg.hubble;
     k.c_fres = (float)(1.881e-7 * std::pow(1. + z, 3.0) * fit_G_H(z));
@@ -160,7 +168,9 @@ CellConsts cell_consts(const sf21_hii_ctx* c, double z) {
     k.k_rec = (float)(1.e-24 * kR1 * std::pow(1.0 + z, kR4) * dvv / kVnorm);
     k.r2 = (float)kR2;
     k.r3 = (float)kR3;
-    k.k_ion = (float)(c->dx * c->dx * c->dx * fit_Qion(z) * c->cfg.fesc);
+    k.k_ion = c->cfg.input_mode == SF21_INPUT_HALO
+                  ? 1.0f
+                  : (float)(c->dx * c->dx * c->dx * fit_Qion(z) * c->cfg.fesc);
   }
   return k;
 }
@@ -227,13 +237,17 @@ void enqueue_fused(sf21_hii_ctx* c, const float* a, const float* b, const float*
   launch(c, "fwd_rows", [&] { launch_fwd_rows(n, a, b, cc, k, c->bal, c->spec, c->tw, s); });
   launch(c, "cols_fwd_j", [&] { launch_cols(n, n, 0, -1, c->spec, c->spec, 1, 0, c->tw, s); });
   launch(c, "fill_map", [&] { launch_fill_i32(c->wmap, -1, n3, s); });
-  const int r_begin = only_radius >= 0 ? only_radius : 0;
-  const int r_end = only_radius >= 0 ? only_radius + 1 : (int)c->radii.size();
-  for (int rb = r_begin; rb < r_end; rb += c->K) {
-    const int kk = std::min(c->K, r_end - rb);
+  // radius batches in ascending radius (from the end of the descending
+  // ladder): each batch's ball bounds exceed the earlier batches', which the
+  // centre-map update relies on
+  const int r_lo = only_radius >= 0 ? only_radius : 0;
+  const int r_hi = only_radius >= 0 ? only_radius + 1 : (int)c->radii.size();
+  for (int re = r_hi; re > r_lo; re -= c->K) {
+    const int rb = std::max(r_lo, re - c->K);
+    const int kk = re - rb;
     launch(c, "filter_i", [&] {
-      launch_filter_i(n, c->spec, c->ubuf, kk, n3, c->wlut + (long)rb * c->mlut, c->mlut, c->tw,
-                      s, c->tile_filter);
+      launch_filter_i(n, c->spec, c->ubuf, kk, n3, c->wlut + (long)rb * c->mlut, c->mlut,
+                      c->d_kappa + rb, c->tw, s, c->tile_filter, c->filter_w);
     });
     launch(c, "cols_inv_j",
            [&] { launch_cols(n, n, 0, +1, c->ubuf, c->ubuf, kk, n3, c->tw, s); });
@@ -331,12 +345,13 @@ void free_all(sf21_hii_ctx* c) {
   for (auto& g : c->graphs)
     if (g.exec) (void)hipGraphExecDestroy(g.exec);
   c->graphs.clear();
-  dev_free(c->tw); dev_free(c->twh); dev_free(c->wlut); dev_free(c->d_d2);
+  dev_free(c->tw); dev_free(c->twh); dev_free(c->wlut); dev_free(c->d_d2); dev_free(c->d_kappa);
   dev_free(c->spec); dev_free(c->ubuf); dev_free(c->wmap); dev_free(c->gbuf);
   dev_free(c->bal_part); dev_free(c->bal); dev_free(c->partials); dev_free(c->neutral_dev);
   dev_free(c->rec); dev_free(c->ion); dev_free(c->fres); dev_free(c->bubblef);
   dev_free(c->box_a); dev_free(c->box_b); dev_free(c->bubble);
   dev_free(c->spec2); dev_free(c->tmp2); dev_free(c->th); dev_free(c->flag_dev);
+  dev_free(c->halo_grid); dev_free(c->halo_buf);
   if (c->flag_host) (void)hipHostFree(c->flag_host);
   if (c->neutral_host) (void)hipHostFree(c->neutral_host);
   c->flag_host = nullptr;
@@ -354,7 +369,11 @@ extern "C" {
 int sf21_hii_create(const sf21_hii_config* cfg, sf21_hii_ctx** out) {
   if (!cfg || !out || !cfg->radii || cfg->n_radii < 1 || !(cfg->box_len > 0)) return SF21_ERR_ARG;
   if (!size_supported(cfg->n, cfg->path)) return SF21_ERR_ARG;
-  if (cfg->input_mode != SF21_INPUT_SFR && cfg->input_mode != SF21_INPUT_DIRECT) return SF21_ERR_ARG;
+  if (cfg->input_mode != SF21_INPUT_SFR && cfg->input_mode != SF21_INPUT_DIRECT &&
+      cfg->input_mode != SF21_INPUT_HALO)
+    return SF21_ERR_ARG;
+  if (cfg->input_mode == SF21_INPUT_HALO && (cfg->n_halo < cfg->n || cfg->n_halo > 4096))
+    return SF21_ERR_ARG;
   if (cfg->path != SF21_PATH_FUSED && cfg->path != SF21_PATH_PLAIN) return SF21_ERR_ARG;
   *out = nullptr;
   int ndev = 0;
@@ -388,6 +407,7 @@ int sf21_hii_create(const sf21_hii_config* cfg, sf21_hii_ctx** out) {
   c->K = cfg->radius_batch > 0 ? cfg->radius_batch : 8;
   if (cfg->filter_tile == 4 || cfg->filter_tile == 8 || cfg->filter_tile == 16)
     c->tile_filter = cfg->filter_tile;
+  c->filter_w = cfg->filter_w == 1 ? 1 : 0;
   if (c->K > (int)c->radii.size()) c->K = (int)c->radii.size();
 
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -406,6 +426,14 @@ int sf21_hii_create(const sf21_hii_config* cfg, sf21_hii_ctx** out) {
         w[r * c->mlut + m] = (float)w_filter(c->dk * std::sqrt((double)m) * c->radii[r]);
     if ((rc = dev_alloc(&c->wlut, w.size()))) goto fail;
     if (hipMemcpy(c->wlut, w.data(), sizeof(float) * w.size(), hipMemcpyHostToDevice) != hipSuccess) {
+      rc = SF21_ERR_HIP;
+      goto fail;
+    }
+    std::vector<float> kap(c->radii.size());
+    for (size_t r = 0; r < c->radii.size(); ++r) kap[r] = (float)(c->dk * c->radii[r]);
+    if ((rc = dev_alloc(&c->d_kappa, kap.size()))) goto fail;
+    if (hipMemcpy(c->d_kappa, kap.data(), sizeof(float) * kap.size(), hipMemcpyHostToDevice) !=
+        hipSuccess) {
       rc = SF21_ERR_HIP;
       goto fail;
     }
@@ -633,6 +661,63 @@ int sf21_hii_reset_stats(sf21_hii_ctx* c) {
   if (!c) return SF21_ERR_ARG;
   c->stats.clear();
   return SF21_OK;
+}
+
+// ---- halo-catalog emissivity (get_HIIbubbles.c:229-259) -----------------
+int sf21_hii_halo_emissivity_dev(sf21_hii_ctx* c, const float* halos, long nhalos, double z,
+                                 float* ion_out) {
+  if (!c || (!halos && nhalos > 0) || nhalos < 0 || !ion_out) return SF21_ERR_ARG;
+  if (c->cfg.input_mode != SF21_INPUT_HALO) return SF21_ERR_ARG;
+  if (hipSetDevice(c->cfg.device) != hipSuccess) return SF21_ERR_HIP;
+  (void)hipGetLastError();
+  const int ng = c->cfg.n_halo, ns = c->n;
+  if (!c->halo_grid) {
+    int rc = dev_alloc(&c->halo_grid, (size_t)ng * ng * ng);
+    if (rc) return rc;
+  }
+  // smooth_box's block size (auxiliary.c:654-656)
+  long sm = (long)roundf((1.0f * ng) / ns);
+  if ((long)ns * sm > ng) sm--;
+  if (sm < 1) return SF21_ERR_ARG;
+  const double dx_h = c->L / ng;
+  const float scale = (float)(std::pow(c->dx / dx_h, 3) * c->cfg.fesc);
+  // Rion (user_functions.c:40-47): c1 (1+z)^c4 / V_norm per redshift
+  const float k_rion = (float)(1.575e39 * std::pow(1.0 + z, 2.27044695e+00) / kVnorm);
+  launch(c, "halo_emissivity", [&] {
+    launch_halo_emissivity(nhalos, halos, ng, ns, (int)sm, k_rion, (float)9.367608e+07, 0.44f,
+                           scale, c->halo_grid, ion_out, c->stream);
+  });
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return SF21_ERR_HIP;
+  if (c->prof) collect(c);
+  return check_hip();
+}
+
+int sf21_hii_run_slice_halo(sf21_hii_ctx* c, const float* delta, const float* halos, long nhalos,
+                            double z, float* xhii, double* neutral) {
+  if (!c || !delta || (!halos && nhalos > 0) || nhalos < 0 || !xhii) return SF21_ERR_ARG;
+  if (c->cfg.input_mode != SF21_INPUT_HALO) return SF21_ERR_ARG;
+  if (hipSetDevice(c->cfg.device) != hipSuccess) return SF21_ERR_HIP;
+  const size_t bytes = sizeof(float) * (size_t)c->n3;
+  float *dd = nullptr, *dion = nullptr, *dx = nullptr;
+  int rc;
+  if (nhalos > c->halo_cap) {
+    dev_free(c->halo_buf);
+    if ((rc = dev_alloc(&c->halo_buf, (size_t)4 * nhalos))) return rc;
+    c->halo_cap = nhalos;
+  }
+  if ((rc = dev_alloc(&dd, c->n3)) || (rc = dev_alloc(&dion, c->n3)) || (rc = dev_alloc(&dx, c->n3))) {
+    dev_free(dd); dev_free(dion); dev_free(dx);
+    return rc;
+  }
+  if (hipMemcpy(dd, delta, bytes, hipMemcpyHostToDevice) != hipSuccess ||
+      (nhalos > 0 && hipMemcpy(c->halo_buf, halos, sizeof(float) * 4 * nhalos,
+                               hipMemcpyHostToDevice) != hipSuccess))
+    rc = SF21_ERR_HIP;
+  if (!rc) rc = sf21_hii_halo_emissivity_dev(c, c->halo_buf, nhalos, z, dion);
+  if (!rc) rc = sf21_hii_run_slice_dev(c, dd, dion, nullptr, z, dx, neutral);
+  if (!rc && hipMemcpy(xhii, dx, bytes, hipMemcpyDeviceToHost) != hipSuccess) rc = SF21_ERR_HIP;
+  dev_free(dd); dev_free(dion); dev_free(dx);
+  return rc;
 }
 
 // ---- FFTW-like transforms ------------------------------------------------

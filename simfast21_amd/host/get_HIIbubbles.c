@@ -6,9 +6,11 @@
  *                   three usage lines and exits 1 (:63-68)
  *   parameters      base_dir/simfast21.ini (Input_variables.c:16-141)
  *   directories     creates Ionization/ and Output_text_files/ (:80-96)
- *   inputs          delta/deltanl_z%.3f_N%ld_L%.1f.dat and
- *                   SFR/sfrd_z%.3f_N%ld_L%.1f.dat (use_SFR = T) per redshift,
- *                   z ascending from the snapped zmin by dz (:206-213, :262-267)
+ *   inputs          delta/deltanl_z%.3f_N%ld_L%.1f.dat and either
+ *                   SFR/sfrd_z%.3f_N%ld_L%.1f.dat (use_SFR = T) or
+ *                   Halos/halonl_z%.3f_N%ld_L%.1f.dat.catalog (use_SFR = F,
+ *                   N = N_halo) per redshift, z ascending from the snapped zmin
+ *                   by dz (:206-213, :236-246, :262-267)
  *   outputs         Ionization/xHII_z%.3f_N%ld_L%.1f.dat for every z, all-zero
  *                   boxes above the cutoff (:480-513); appends zsim.txt and
  *                   x_HI_N%ld_L%.1f.dat (:515-528)
@@ -17,8 +19,7 @@
  * Differences (deliberate):
  *   - a short input file is an error here; the reference does not check
  *     fread and would run on stale buffer contents (:212, :266);
- *   - halo-catalog emissivity (use_SFR = F, :229-259) is not available yet:
- *     the program says so and exits 1 before touching the GPU;
+ *   - a short halo catalog is an error as well;
  *   - the bubble finder runs on the GPU (fused exact path); nthreads is read
  *     but only sizes the host I/O.
  */
@@ -57,6 +58,28 @@ static int read_box(const char *path, float *buf, long n3) {
   return got == (size_t)n3 ? SF21_OK : SF21_ERR_IO;
 }
 
+/* Halo catalog: long count, then count x {float Mass, x, y, z}
+ * (output_files.txt:10-14, auxiliary.h:13-17). */
+static float *read_catalog(const char *path, long *count) {
+  FILE *f = fopen(path, "rb");
+  float *h;
+  long n = 0;
+  if (!f) return NULL;
+  if (fread(&n, sizeof(long), 1, f) != 1 || n < 0) {
+    fclose(f);
+    return NULL;
+  }
+  h = (float *)malloc(sizeof(float) * 4 * (size_t)(n > 0 ? n : 1));
+  if (!h || (n > 0 && fread(h, 4 * sizeof(float), (size_t)n, f) != (size_t)n)) {
+    free(h);
+    fclose(f);
+    return NULL;
+  }
+  fclose(f);
+  *count = n;
+  return h;
+}
+
 static void write_box(const char *path, const float *buf, long n3) {
   FILE *f = fopen(path, "wb");
   if (!f) {
@@ -74,7 +97,8 @@ int main(int argc, char *argv[]) {
   sf21_params p;
   char msg[512], fname[4096];
   double *radii, *xhi, z, neutral, bfactor, lmpc;
-  float *delta, *sfrd, *xhii;
+  float *delta, *sfrd, *xhii, *halos = NULL;
+  long nhalos = 0;
   long n3, i;
   int nr, nmain, cap, iz, rc;
   sf21_hii_ctx *ctx = NULL;
@@ -89,10 +113,6 @@ int main(int argc, char *argv[]) {
   rc = sf21_params_read(argv[1], &p, msg, sizeof(msg));
   if (rc != SF21_OK) {
     printf("%s", msg);
-    exit(1);
-  }
-  if (!p.use_sfr) {
-    printf("Halo-catalog emissivity (use_SFR = F) is not supported by this build; set use_SFR = T\n");
     exit(1);
   }
   make_dir(argv[1], "Ionization", "Creating Ionization directory\n");
@@ -133,10 +153,23 @@ int main(int argc, char *argv[]) {
       printf("\nError reading deltanl file... Check path or if the file exists...");
       exit(1);
     }
-    snprintf(fname, sizeof(fname), "%s/SFR/sfrd_z%.3f_N%ld_L%.1f.dat", argv[1], z, p.n, lmpc);
-    if (read_box(fname, sfrd, n3) != SF21_OK) {
-      printf("\nError reading %s file... Check path or if the file exists...", fname);
-      exit(1);
+    if (p.use_sfr) {
+      snprintf(fname, sizeof(fname), "%s/SFR/sfrd_z%.3f_N%ld_L%.1f.dat", argv[1], z, p.n, lmpc);
+      if (read_box(fname, sfrd, n3) != SF21_OK) {
+        printf("\nError reading %s file... Check path or if the file exists...", fname);
+        exit(1);
+      }
+    } else {
+      snprintf(fname, sizeof(fname), "%s/Halos/halonl_z%.3f_N%ld_L%.1f.dat.catalog", argv[1], z,
+               p.n_halo, lmpc);
+      free(halos);
+      halos = read_catalog(fname, &nhalos);
+      if (!halos) {
+        printf("\nError reading %s file... Check path or if the file exists...", fname);
+        exit(1);
+      }
+      printf("Reading %ld halos...\n", nhalos);
+      fflush(0);
     }
     if (!ctx) {
       sf21_hii_config cfg;
@@ -145,7 +178,8 @@ int main(int argc, char *argv[]) {
       cfg.box_len = p.box_len;
       cfg.n_radii = nr;
       cfg.radii = radii;
-      cfg.input_mode = SF21_INPUT_SFR;
+      cfg.input_mode = p.use_sfr ? SF21_INPUT_SFR : SF21_INPUT_HALO;
+      cfg.n_halo = (int)p.n_halo;
       cfg.path = SF21_PATH_FUSED;
       cfg.hubble = p.hubble;
       cfg.fesc = p.fesc;
@@ -157,7 +191,10 @@ int main(int argc, char *argv[]) {
     }
     printf("Going to smaller R cycle...\n");
     fflush(0);
-    rc = sf21_hii_run_slice(ctx, delta, sfrd, NULL, z, xhii, &neutral);
+    if (p.use_sfr)
+      rc = sf21_hii_run_slice(ctx, delta, sfrd, NULL, z, xhii, &neutral);
+    else
+      rc = sf21_hii_run_slice_halo(ctx, delta, halos, nhalos, z, xhii, &neutral);
     if (rc != SF21_OK) {
       printf("Bubble finder failed: %s\n", sf21_strerror(rc));
       exit(1);
@@ -201,5 +238,6 @@ int main(int argc, char *argv[]) {
   free(xhii);
   free(xhi);
   free(radii);
+  free(halos);
   return 0;
 }

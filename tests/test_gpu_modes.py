@@ -148,3 +148,61 @@ def test_cli_end_to_end_vs_oracle_cli(tmp_path, torch_cuda):
         o = np.fromfile(dirs["oracle"] / "Ionization" / f, np.float32)
         assert g.size == o.size == 64 ** 3
         assert abs(g.mean(dtype=np.float64) - o.mean(dtype=np.float64)) < 1e-5
+
+
+def halo_catalog(n_halo, count, seed):
+    rng = np.random.default_rng(seed)
+    m = 10 ** rng.uniform(7.5, 10.5, count)
+    pos = rng.uniform(0, n_halo, (count, 3))
+    pos[:5] = [[0.0, 0.0, 0.0], [n_halo - 0.25, 3.5, 7.0], [-0.3, 2.2, 1.0], [1.0, -0.6, n_halo - 1e-3],
+               [5.5, 6.5, -0.75]]  # edges and the reference's negative-remainder branch
+    return np.column_stack([m, pos]).astype(np.float32)
+
+
+@pytest.mark.gpu
+def test_halo_emissivity_vs_oracle(torch_cuda):
+    """CIC of Rion + smooth_box + (dx_s/dx_h)^3 fesc (get_HIIbubbles.c:229-259)
+    against the oracle's FP64-accumulated restatement: FP32 atomics and FP32
+    fits differ at the 1e-5 level."""
+    torch = torch_cuda
+    n, n_halo, L_h, z, fesc = 64, 128, 70.0, 8.0, 0.25
+    cat = halo_catalog(n_halo, 20000, 5)
+    radii, _ = sf.ladder(0.3 * L_h, L_h / n, 20)
+    ctx = sf.HiiContext(n, L_h, radii, sf.INPUT_HALO, sf.PATH_FUSED, hubble=0.7, fesc=fesc, n_halo=n_halo)
+    ion = ctx.halo_emissivity(torch.from_numpy(cat).cuda(), z).cpu().numpy()
+    ref = np.zeros(n ** 3, np.float32)
+    orc.lib().orc_halo_emissivity(len(cat), cat.ctypes.data, z, n_halo, n, L_h / n_halo, L_h / n, fesc,
+                                  ref.ctypes.data)
+    assert ref.max() > 0
+    err = np.abs(ion - ref) / ref.max()
+    print(f"halo emissivity: max |err|/max {err.max():.2e}")
+    assert err.max() < 2e-5
+
+
+@pytest.mark.gpu
+def test_cli_halo_mode_vs_oracle_cli(tmp_path, torch_cuda):
+    ini = INI.replace("use_SFR = T", "use_SFR = F")
+    dirs = {}
+    for who in ("gpu", "oracle"):
+        d = tmp_path / who
+        (d / "delta").mkdir(parents=True)
+        (d / "Halos").mkdir()
+        (d / "simfast21.ini").write_text(ini)
+        for k, z in enumerate((7.0, 8.0, 9.0)):
+            c = sfr_case(n=64, z=z, seed=40 + k)
+            c["delta"].astype(np.float32).tofile(d / "delta" / f"deltanl_z{z:.3f}_N64_L100.0.dat")
+            cat = halo_catalog(128, 3000 * (k + 1), 50 + k)
+            with open(d / "Halos" / f"halonl_z{z:.3f}_N128_L100.0.dat.catalog", "wb") as f:
+                np.array([len(cat)], np.int64).tofile(f)
+                cat.tofile(f)
+        dirs[who] = d
+    r = subprocess.run([str(sf.CLI_PATH), str(dirs["gpu"])], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "Reading 3000 halos..." in r.stdout
+    env = dict(os.environ, ORC_PRECISION="64", ORC_DILATION="1")
+    r2 = subprocess.run([orc.CLI, str(dirs["oracle"])], capture_output=True, text=True, env=env)
+    assert r2.returncode == 0, r2.stdout
+    g = np.array((dirs["gpu"] / "Output_text_files" / "x_HI_N64_L100.0.dat").read_text().split(), float)
+    o = np.array((dirs["oracle"] / "Output_text_files" / "x_HI_N64_L100.0.dat").read_text().split(), float)
+    print("x_HI gpu", g, "oracle", o)
+    assert np.allclose(g, o, atol=1e-5, rtol=0)

@@ -108,7 +108,7 @@ def test_create_without_gpu_fails_cleanly():
     if _have_gpu():
         pytest.skip("a HIP device is present")
     radii = (ctypes.c_double * 2)(4.0, 2.0)
-    cfg = sf.HiiConfig(64, 100.0, 2, radii, sf.INPUT_DIRECT, sf.PATH_FUSED, 0.7, 1.0, 0, 0, 0)
+    cfg = sf.HiiConfig(64, 100.0, 2, radii, sf.INPUT_DIRECT, sf.PATH_FUSED, 0.7, 1.0, 0, 0, 0, 0, 0)
     h = ctypes.c_void_p()
     assert sf._lib().sf21_hii_create(ctypes.byref(cfg), ctypes.byref(h)) == sf.SF21_ERR_HIP
     cfg.n = 100  # unsupported size is rejected before any device work

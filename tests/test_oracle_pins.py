@@ -174,3 +174,31 @@ def test_filtered_vs_numpy_independent():
         ref = np.fft.irfftn(np.fft.rfftn(box.reshape(n, n, n).astype(np.float64)) * W,
                             s=(n, n, n), axes=(0, 1, 2)) * n ** 3
         assert abs(got.reshape(n, n, n) - ref).max() / abs(ref).max() < 1e-9
+
+
+def test_cic_and_smooth_box_known_answers():
+    """auxiliary.c:761-808 and :643-682: an integer position deposits the
+    whole value in one cell, a half-integer position 1/8 in each of 8 cells,
+    mass is conserved with periodic wrap, a negative remainder is mirrored
+    (the reference's x1 < 0 branch), and smooth_box averages sm^3 blocks."""
+    L = orc.lib()
+    L.orc_cic.argtypes = [ctypes.c_float] * 4 + [ctypes.c_void_p, ctypes.c_long]
+    L.orc_smooth_box.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long, ctypes.c_long]
+    n = 8
+    g = np.zeros(n ** 3, np.float32)
+    L.orc_cic(3.0, 4.0, 5.0, 2.0, g.ctypes.data, n)
+    assert g[(3 * n + 4) * n + 5] == 2.0 and g.sum() == 2.0
+    g[:] = 0
+    L.orc_cic(7.5, 0.5, 3.5, 8.0, g.ctypes.data, n)
+    nz = np.nonzero(g)[0]
+    assert len(nz) == 8 and np.allclose(g[nz], 1.0)
+    assert g[(7 * n + 0) * n + 3] == 1.0 and g[(0 * n + 1) * n + 4] == 1.0  # wraps 7 -> 0
+    g[:] = 0
+    L.orc_cic(-0.25, 2.0, 2.0, 1.0, g.ctypes.data, n)
+    # trunc(-0.25) = 0, remainder -0.25 -> cell -1 (= 7) with 1 - 0.25, cell 0 with 0.25
+    assert abs(g[(7 * n + 2) * n + 2] - 0.75) < 1e-7 and abs(g[(0 * n + 2) * n + 2] - 0.25) < 1e-7
+    box = np.arange(16 ** 3, dtype=np.float32)
+    out = np.zeros(8 ** 3, np.float32)
+    L.orc_smooth_box(box.ctypes.data, out.ctypes.data, 16, 8)
+    ref = box.reshape(8, 2, 8, 2, 8, 2).mean(axis=(1, 3, 5)).ravel()
+    assert np.allclose(out, ref)
