@@ -105,6 +105,16 @@ int sf21_hii_halo_emissivity_dev(sf21_hii_ctx *ctx, const float *halos, long n_h
 int sf21_hii_run_slice_halo(sf21_hii_ctx *ctx, const float *delta, const float *halos,
                             long n_halos, double z, float *xhii, double *neutral);
 
+/* Spherically averaged power spectrum of a real n^3 box (device pointer),
+ * normalised as tools/power3d.c:172-191 does for a top-hat window
+ * (P = <|F_k|^2> dl^3 / n^3 over half-spectrum entries, dl = box_len/n), in
+ * nbins (<= 64) linear bins of |k|/dk over [1, n/2] (width (n/2-1)/nbins,
+ * last bin closed; k = 0 and corners beyond n/2 dropped).  Sums in FP64.
+ * Host outputs: pk, kmean (mean |k| per bin), counts (modes per bin); any
+ * may be NULL.  n >= 64.  Uses the context's spectrum buffer. */
+int sf21_hii_power_spectrum_dev(sf21_hii_ctx *ctx, const float *field, int nbins, double *pk,
+                                double *kmean, double *counts);
+
 /* Inspection for tests (fused path): per-cell ball bound d2max(R) of the
  * largest radius at which the cell was a bubble centre in the last slice, -1
  * if never (host output, n^3 ints). */

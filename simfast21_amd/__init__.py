@@ -114,6 +114,7 @@ def _lib():
     L.sf21_hii_profile.argtypes = [vp, i]
     L.sf21_hii_kernel_stats.argtypes = [vp, i, ctypes.c_char_p, i, ctypes.POINTER(ctypes.c_long), dp]
     L.sf21_hii_reset_stats.argtypes = [vp]
+    L.sf21_hii_power_spectrum_dev.argtypes = [vp, vp, i, vp, vp, vp]
     L.sf21_hii_halo_emissivity_dev.argtypes = [vp, vp, ctypes.c_long, d, vp]
     L.sf21_hii_run_slice_halo.argtypes = [vp, vp, vp, ctypes.c_long, d, vp, dp]
     L.sf21_fft_plan_3d.argtypes = [i, i, ctypes.POINTER(vp)]
@@ -258,6 +259,18 @@ class HiiContext:
         _check(_lib().sf21_hii_halo_emissivity_dev(self._h, _ptr(halos), int(halos.shape[0]), float(z),
                                                    _ptr(out)), "sf21_hii_halo_emissivity_dev")
         return out
+
+    def power_spectrum(self, field, nbins: int = 20):
+        """Binned P(k) of a real device box: (pk, kmean, counts) numpy arrays."""
+        import torch
+        pk = np.zeros(nbins)
+        km = np.zeros(nbins)
+        ct = np.zeros(nbins)
+        torch.cuda.current_stream().synchronize()
+        _check(_lib().sf21_hii_power_spectrum_dev(self._h, _ptr(field), int(nbins), pk.ctypes.data,
+                                                  km.ctypes.data, ct.ctypes.data),
+               "sf21_hii_power_spectrum_dev")
+        return pk, km, ct
 
     def weight_map(self) -> np.ndarray:
         out = np.empty(self.n ** 3, np.int32)

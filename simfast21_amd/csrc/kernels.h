@@ -64,6 +64,11 @@ int clamp_partials_count(long n3);
 void launch_clamp_sum(long n3, float* bubblef, const float* fresid, double* partials,
                       hipStream_t s);
 
+// ---- binned power spectrum ----
+int pk_partials_count(int n);
+void launch_power_spectrum(int n, const float2* spec, const short* bin_of_m, const float* kabs_of_m,
+                           int nbins, double norm, double* part, double* out, hipStream_t s);
+
 // ---- halo-catalog emissivity ----
 void launch_halo_emissivity(long nhalos, const float* halos, int ng, int ns, int sm, float k_rion,
                             float c2, float c3, float scale, float* grid, float* out,

@@ -364,3 +364,72 @@ void launch_clamp_sum(long n3, float* bubblef, const float* fresid, double* part
 }
 
 }  // namespace sf21
+
+namespace sf21 {
+
+// ------------------------------------------------------------------------
+// Binned power spectrum of a half spectrum (tools/power3d.c:172-191
+// normalisation): per block FP64 sums of |F|^2, |k| and counts per bin in
+// LDS, written as partials and reduced in a fixed order.
+// ------------------------------------------------------------------------
+constexpr int kPkMaxBins = 64;
+
+__global__ __launch_bounds__(256) void k_pk_partial(int n, const float2* __restrict__ spec,
+                                                    const short* __restrict__ bin_of_m,
+                                                    const float* __restrict__ kabs_of_m,
+                                                    int nbins, double* __restrict__ part) {
+  __shared__ double s_p[kPkMaxBins], s_k[kPkMaxBins], s_c[kPkMaxBins];
+  for (int b = threadIdx.x; b < nbins; b += blockDim.x) s_p[b] = s_k[b] = s_c[b] = 0.0;
+  __syncthreads();
+  const int h = n / 2 + 1;
+  const long nh = (long)n * n * h;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < nh;
+       i += (long)gridDim.x * blockDim.x) {
+    const int q = (int)(i % h);
+    const long r = i / h;
+    const int m = fold_sq((int)(r / n), n) + fold_sq((int)(r % n), n) + q * q;
+    const int b = bin_of_m[m];
+    if (b < 0) continue;
+    const float2 f = spec[i];
+    atomicAdd(&s_p[b], (double)f.x * f.x + (double)f.y * f.y);
+    atomicAdd(&s_k[b], (double)kabs_of_m[m]);
+    atomicAdd(&s_c[b], 1.0);
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < nbins; b += blockDim.x) {
+    part[(3L * blockIdx.x) * nbins + b] = s_p[b];
+    part[(3L * blockIdx.x + 1) * nbins + b] = s_k[b];
+    part[(3L * blockIdx.x + 2) * nbins + b] = s_c[b];
+  }
+}
+
+__global__ void k_pk_final(const double* __restrict__ part, int nblocks, int nbins,
+                           double norm, double* __restrict__ out) {
+  const int b = threadIdx.x;
+  if (b >= nbins) return;
+  double p = 0.0, k = 0.0, c = 0.0;
+  for (int i = 0; i < nblocks; ++i) {
+    p += part[(3L * i) * nbins + b];
+    k += part[(3L * i + 1) * nbins + b];
+    c += part[(3L * i + 2) * nbins + b];
+  }
+  out[b] = c > 0 ? p / c * norm : 0.0;
+  out[nbins + b] = c > 0 ? k / c : 0.0;
+  out[2 * nbins + b] = c;
+}
+
+int pk_partials_count(int n) {
+  long nh = (long)n * n * (n / 2 + 1);
+  long b = (nh + 255) / 256;
+  return (int)(b > 2048 ? 2048 : b);
+}
+
+void launch_power_spectrum(int n, const float2* spec, const short* bin_of_m, const float* kabs_of_m,
+                           int nbins, double norm, double* part, double* out, hipStream_t s) {
+  const int nb = pk_partials_count(n);
+  hipLaunchKernelGGL(k_pk_partial, dim3(nb), dim3(256), 0, s, n, spec, bin_of_m, kabs_of_m, nbins,
+                     part);
+  hipLaunchKernelGGL(k_pk_final, dim3(1), dim3(kPkMaxBins), 0, s, part, nb, nbins, norm, out);
+}
+
+}  // namespace sf21

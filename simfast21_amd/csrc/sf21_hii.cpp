@@ -103,6 +103,12 @@ struct sf21_hii_ctx {
   float2 *spec2 = nullptr, *tmp2 = nullptr, *th = nullptr;
   int* flag_dev = nullptr;
   int* flag_host = nullptr;
+  // power spectrum
+  short* pk_bin = nullptr;
+  float* pk_kabs = nullptr;
+  double* pk_part = nullptr;
+  double* pk_out = nullptr;
+  int pk_nbins = 0;
   // halo mode
   float* halo_grid = nullptr;   // N_halo^3 deposit grid
   float* halo_buf = nullptr;    // catalog upload (host entry point)
@@ -352,6 +358,7 @@ void free_all(sf21_hii_ctx* c) {
   dev_free(c->box_a); dev_free(c->box_b); dev_free(c->bubble);
   dev_free(c->spec2); dev_free(c->tmp2); dev_free(c->th); dev_free(c->flag_dev);
   dev_free(c->halo_grid); dev_free(c->halo_buf);
+  dev_free(c->pk_bin); dev_free(c->pk_kabs); dev_free(c->pk_part); dev_free(c->pk_out);
   if (c->flag_host) (void)hipHostFree(c->flag_host);
   if (c->neutral_host) (void)hipHostFree(c->neutral_host);
   c->flag_host = nullptr;
@@ -661,6 +668,64 @@ int sf21_hii_reset_stats(sf21_hii_ctx* c) {
   if (!c) return SF21_ERR_ARG;
   c->stats.clear();
   return SF21_OK;
+}
+
+// ---- binned power spectrum ------------------------------------------------
+int sf21_hii_power_spectrum_dev(sf21_hii_ctx* c, const float* field, int nbins, double* pk,
+                                double* kmean, double* counts) {
+  if (!c || !field || nbins < 1 || nbins > 64 || c->n < 64) return SF21_ERR_ARG;
+  if (hipSetDevice(c->cfg.device) != hipSuccess) return SF21_ERR_HIP;
+  (void)hipGetLastError();
+  int rc;
+  const int n = c->n, hh = n / 2;
+  if (c->pk_nbins != nbins) {
+    // bin of every integer |k|^2 (in dk units): linear bins of |k| over
+    // [1, n/2], width (n/2 - 1)/nbins, the last closed; k = 0 and the
+    // corners beyond n/2 are dropped (-1)
+    const long mmax = 3L * hh * hh;
+    std::vector<short> bins(mmax + 1);
+    std::vector<float> kab(mmax + 1);
+    const double width = (hh - 1) / (double)nbins;
+    for (long m = 0; m <= mmax; ++m) {
+      const double kq = std::sqrt((double)m);
+      int b = -1;
+      if (m >= 1 && kq <= hh) {
+        b = (int)std::floor((kq - 1.0) / width);
+        if (b >= nbins) b = nbins - 1;
+      }
+      bins[m] = (short)b;
+      kab[m] = (float)(kq * c->dk);
+    }
+    dev_free(c->pk_bin); dev_free(c->pk_kabs); dev_free(c->pk_part); dev_free(c->pk_out);
+    if ((rc = dev_alloc(&c->pk_bin, bins.size())) || (rc = dev_alloc(&c->pk_kabs, kab.size())) ||
+        (rc = dev_alloc(&c->pk_part, (size_t)3 * nbins * pk_partials_count(n))) ||
+        (rc = dev_alloc(&c->pk_out, (size_t)3 * nbins)))
+      return rc;
+    if (hipMemcpy(c->pk_bin, bins.data(), sizeof(short) * bins.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->pk_kabs, kab.data(), sizeof(float) * kab.size(), hipMemcpyHostToDevice) != hipSuccess)
+      return SF21_ERR_HIP;
+    c->pk_nbins = nbins;
+  }
+  // the half spectrum goes to the context's spectrum buffer (n^3 >= n*n*(n/2+1))
+  r2c3d(c, field, c->spec);
+  const double dl = c->dx;
+  const double norm = dl * dl * dl / ((double)c->n3);
+  launch(c, "power_spectrum", [&] {
+    launch_power_spectrum(n, c->spec, c->pk_bin, c->pk_kabs, nbins, norm, c->pk_part, c->pk_out,
+                          c->stream);
+  });
+  std::vector<double> host(3 * nbins);
+  if (hipMemcpyAsync(host.data(), c->pk_out, sizeof(double) * 3 * nbins, hipMemcpyDeviceToHost,
+                     c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess)
+    return SF21_ERR_HIP;
+  for (int b = 0; b < nbins; ++b) {
+    if (pk) pk[b] = host[b];
+    if (kmean) kmean[b] = host[nbins + b];
+    if (counts) counts[b] = host[2 * nbins + b];
+  }
+  if (c->prof) collect(c);
+  return check_hip();
 }
 
 // ---- halo-catalog emissivity (get_HIIbubbles.c:229-259) -----------------

@@ -206,3 +206,30 @@ def test_cli_halo_mode_vs_oracle_cli(tmp_path, torch_cuda):
     o = np.array((dirs["oracle"] / "Output_text_files" / "x_HI_N64_L100.0.dat").read_text().split(), float)
     print("x_HI gpu", g, "oracle", o)
     assert np.allclose(g, o, atol=1e-5, rtol=0)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n", [64, 128])
+def test_power_spectrum_vs_oracle(torch_cuda, n):
+    """20-bin P(k) (tools/power3d.c normalisation) against the oracle's FP64
+    restatement and numpy's rfftn."""
+    torch = torch_cuda
+    side = 100.0
+    g = synth.gaussian_field(n, -2.5, 1.2, seed=9).astype(np.float64)
+    ctx = sf.HiiContext(n, side, [10.0, 5.0], sf.INPUT_DIRECT, sf.PATH_FUSED)
+    pk, km, ct = ctx.power_spectrum(torch.from_numpy(g.astype(np.float32).ravel()).cuda(), 20)
+    rp, rk, rc = np.zeros(20), np.zeros(20), np.zeros(20, np.int64)
+    orc.lib().orc_power_spectrum(n, side, np.ascontiguousarray(g).ctypes.data, 20, rp.ctypes.data,
+                                 rk.ctypes.data, rc.ctypes.data)
+    assert np.array_equal(ct.astype(np.int64), rc)
+    assert np.allclose(km, rk, rtol=1e-6)
+    assert np.allclose(pk, rp, rtol=2e-5), np.max(np.abs(pk / rp - 1))
+    # numpy restatement of the same definition
+    F = np.fft.rfftn(g)
+    kx = np.fft.fftfreq(n) * n
+    kq = np.sqrt(kx[:, None, None] ** 2 + kx[None, :, None] ** 2 + np.arange(n // 2 + 1)[None, None, :] ** 2)
+    sel = (kq >= 1) & (kq <= n // 2)
+    b = np.minimum(np.floor((kq[sel] - 1) / ((n // 2 - 1) / 20)).astype(int), 19)
+    p2 = np.bincount(b, weights=np.abs(F[sel]) ** 2, minlength=20) / np.bincount(b, minlength=20)
+    p2 *= (side / n) ** 3 / n ** 3
+    assert np.allclose(rp, p2, rtol=1e-10)
