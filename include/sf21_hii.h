@@ -115,6 +115,33 @@ int sf21_hii_run_slice_halo(sf21_hii_ctx *ctx, const float *delta, const float *
 int sf21_hii_power_spectrum_dev(sf21_hii_ctx *ctx, const float *field, int nbins, double *pk,
                                 double *kmean, double *counts);
 
+/* ---- Differentiable mode -------------------------------------------------
+ * No counterpart in the reference (SimFast21 has no gradients); it wraps the
+ * bubble finder of get_HIIbubbles.c:277-479 for inference work.  Context:
+ * input_mode SF21_INPUT_DIRECT, path SF21_PATH_FUSED, n >= 64.  Inputs
+ * (device, n^3 floats): g, a Gaussian field, and emis, an emissivity
+ * pattern; zeta scales it.  The model:
+ *   Delta = exp(g)/<exp(g)>, rec = Delta^2, ion = zeta*emis, no residual
+ *   neutral fraction; x = the exact bubble finder on (rec, ion);
+ *   T = (1 - x) Delta; observables <x> and P_b(T) in nbins bins as
+ *   sf21_hii_power_spectrum_dev defines them.
+ * diff_forward runs the exact model: x_out (device, nullable), *mean_x and
+ * pk (host, nbins, nullable).
+ * diff_backward returns the gradient of L = dl_dx <x~> + sum_b dl_dpk[b] P_b
+ * of the tau-relaxed model (the centre test min(I_R - R_R, I_R) >= 0 becomes
+ * softplus(s_R/tau) with s_R = min(I_R - R_R, I_R)/max(R_R, 1e-3 <rec>);
+ * the union of balls becomes x~ = q + (1 - q)(1 - exp(-sum_R B_R * phi_R)),
+ * B_R the exact ball, q the quick fill) with respect to g (grad_g, device
+ * n^3) and zeta (*grad_zeta), plus the relaxed <x~> and P_b (host,
+ * nullable).  Per-radius fields are recomputed in the backward, never
+ * stored; tau -> 0 gives back the exact forward.  Sums in FP64. */
+int sf21_hii_diff_forward_dev(sf21_hii_ctx *ctx, const float *g, const float *emis, double zeta,
+                              int nbins, float *x_out, double *mean_x, double *pk);
+int sf21_hii_diff_backward_dev(sf21_hii_ctx *ctx, const float *g, const float *emis,
+                               double zeta, double tau, int nbins, double dl_dx,
+                               const double *dl_dpk, float *grad_g, double *grad_zeta,
+                               double *relaxed_mean_x, double *relaxed_pk);
+
 /* Inspection for tests (fused path): per-cell ball bound d2max(R) of the
  * largest radius at which the cell was a bubble centre in the last slice, -1
  * if never (host output, n^3 ints). */

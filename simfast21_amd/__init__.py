@@ -116,6 +116,8 @@ def _lib():
     L.sf21_hii_reset_stats.argtypes = [vp]
     L.sf21_hii_power_spectrum_dev.argtypes = [vp, vp, i, vp, vp, vp]
     L.sf21_hii_halo_emissivity_dev.argtypes = [vp, vp, ctypes.c_long, d, vp]
+    L.sf21_hii_diff_forward_dev.argtypes = [vp, vp, vp, d, i, vp, dp, vp]
+    L.sf21_hii_diff_backward_dev.argtypes = [vp, vp, vp, d, d, i, d, vp, vp, dp, dp, vp]
     L.sf21_hii_run_slice_halo.argtypes = [vp, vp, vp, ctypes.c_long, d, vp, dp]
     L.sf21_fft_plan_3d.argtypes = [i, i, ctypes.POINTER(vp)]
     L.sf21_fft_execute.argtypes = [vp, vp, vp, vp]
@@ -271,6 +273,39 @@ class HiiContext:
                                                   km.ctypes.data, ct.ctypes.data),
                "sf21_hii_power_spectrum_dev")
         return pk, km, ct
+
+    def diff_forward(self, g, emis, zeta: float, nbins: int = 20, x_out=None):
+        """Exact forward of the differentiable model on device boxes g, emis:
+        (x, mean_x, pk).  See sf21_hii_diff_forward_dev."""
+        import torch
+        if x_out is None:
+            x_out = torch.empty_like(g)
+        pk = np.zeros(nbins)
+        mx = ctypes.c_double()
+        torch.cuda.current_stream().synchronize()
+        _check(_lib().sf21_hii_diff_forward_dev(self._h, _ptr(g), _ptr(emis), float(zeta), int(nbins),
+                                                _ptr(x_out), ctypes.byref(mx), pk.ctypes.data),
+               "sf21_hii_diff_forward_dev")
+        return x_out, mx.value, pk
+
+    def diff_backward(self, g, emis, zeta: float, tau: float, dl_dx: float, dl_dpk,
+                      nbins: int = 20, grad_out=None):
+        """Relaxed-model gradient of dl_dx <x~> + sum dl_dpk P_b:
+        (grad_g (device), grad_zeta, relaxed mean x~, relaxed pk)."""
+        import torch
+        if grad_out is None:
+            grad_out = torch.empty_like(g)
+        gp = np.ascontiguousarray(np.zeros(nbins) if dl_dpk is None else dl_dpk, np.float64)
+        pk = np.zeros(nbins)
+        gz = ctypes.c_double()
+        mx = ctypes.c_double()
+        torch.cuda.current_stream().synchronize()
+        _check(_lib().sf21_hii_diff_backward_dev(self._h, _ptr(g), _ptr(emis), float(zeta), float(tau),
+                                                 int(nbins), float(dl_dx), gp.ctypes.data,
+                                                 _ptr(grad_out), ctypes.byref(gz), ctypes.byref(mx),
+                                                 pk.ctypes.data),
+               "sf21_hii_diff_backward_dev")
+        return grad_out, gz.value, mx.value, pk
 
     def weight_map(self) -> np.ndarray:
         out = np.empty(self.n ** 3, np.int32)

@@ -69,6 +69,58 @@ int pk_partials_count(int n);
 void launch_power_spectrum(int n, const float2* spec, const short* bin_of_m, const float* kabs_of_m,
                            int nbins, double norm, double* part, double* out, hipStream_t s);
 
+// ---- differentiable mode (exact forward, tau-relaxed adjoint) ----
+// slots of the FP64 device state array
+enum DiffSlot {
+  DS_SUM_EXP = 0,  // sum exp(g)
+  DS_SUM_REC,      // sum rec (rho = 1e-3 sum rec / N)
+  DS_SUM_X,        // sum x (exact forward)
+  DS_SUM_XT,       // sum x~ (relaxed model)
+  DS_RHO_BAR,      // dL/drho
+  DS_ZETA_BAR,     // sum ionbar E = dL/dzeta
+  DS_S,            // sum Deltabar Delta
+  DS_COUNT
+};
+bool diff_size_supported(int n);
+int diff_partials_count(long n3);
+int rows_blocks(int n);
+void launch_sum_exp(long n3, const float* g, double* part, double* st, hipStream_t s);
+void launch_make_fields(long n3, const float* g, const float* emis, float zeta, double* st,
+                        float* delta1, float* rec, float* ion, double* part, hipStream_t s);
+void launch_t_field(long n3, const float* x, const float* delta1, float* t, double* part,
+                    double* st, hipStream_t s);
+void launch_ball_box(int n, int d2, float2* out, hipStream_t s);
+void launch_ball_fold(int n, const float2* spec, float* table, hipStream_t s);
+void launch_rows_phi(int n, float2* u, int nrad, long ustride, const float* bal, const double* st,
+                     float inv_tau, const float2* tw, hipStream_t s);
+void launch_rows_seed(int n, float2* u, const float2* v, int nrad, long ustride, long vstride,
+                      const float* bal, const double* st, float inv_tau, double* rho_acc,
+                      const float2* tw, hipStream_t s);
+void launch_ball_i(int n, bool acc, const float2* in, long istride, float2* out, long ostride,
+                   int nrad, const float* bt, const float2* tw, hipStream_t s);
+void launch_filter_acc_i(int n, const float2* in, long istride, int nrad, const float* wlut,
+                         long mlut, float2* acc, const float2* tw, hipStream_t s);
+void launch_relaxed_state(long n3, const float2* cacc, const float* rec, const float* ion,
+                          const float* delta1, float* xt, float* t, float* fq, float* fa,
+                          double* part, double* st, hipStream_t s);
+void launch_pk_adjoint(int n, float2* spec, const short* bin_of_m, const double* coef,
+                       hipStream_t s);
+void launch_pk_coef(const double* gp, const double* pk, int nbins, double norm, double* coef,
+                    hipStream_t s);
+void launch_seed_a(long n3, const float* tbar, const float* delta1, const float* xt,
+                   const float* fq, const float* fa, float gx_n, float2* abar, float* qbar,
+                   float* dbar, hipStream_t s);
+void launch_sum_rho(const double* acc, int np, double* st, hipStream_t s);
+void launch_grad_cells(long n3, const float2* gacc, const float* qbar, const float* rec,
+                       const float* ion, const float* emis, const float* delta1,
+                       const float* dbar, double* st, float* dd_out, double* part,
+                       hipStream_t s);
+void launch_grad_g(long n3, const float* delta1, const float* dd, const double* st, float* gout,
+                   hipStream_t s);
+void launch_fill_c(float2* p, long count, hipStream_t s);
+void launch_fill_d(double* p, long count, hipStream_t s);
+void launch_to_complex(long n3, const float* in, float2* out, hipStream_t s);
+
 // ---- halo-catalog emissivity ----
 void launch_halo_emissivity(long nhalos, const float* halos, int ng, int ns, int sm, float k_rion,
                             float c2, float c3, float scale, float* grid, float* out,

@@ -1,0 +1,752 @@
+// Kernels of the differentiable mode.  The forward stays the exact bubble
+// finder; the backward differentiates a tau-relaxed model of it:
+//   Delta = exp(g)/<exp(g)>,  rec = Delta^2,  ion = zeta E
+//   s_R   = min(I_R - R_R, I_R) / max(R_R, rho),  rho = 1e-3 <rec>
+//           (R_R, I_R: rec and ion filtered by the top hat W_R; the sign of
+//            s_R is the centre test of get_HIIbubbles.c:339-349)
+//   A     = sum_R B_R (*) softplus(s_R / tau)   (B_R: the lattice ball of
+//           radius R, (*) periodic convolution)
+//   x~    = q + (1 - q)(1 - exp(-A))            (q: quick fill, :281-288)
+//   T     = (1 - x~) Delta,   observables <x~> and the binned P(k) of T.
+//
+// Box layout as in the bubble finder: index (i*n + j)*n + p.  "(p,j)-space"
+// means transformed along p and j and real along i (the layout of the
+// forward spectrum).  Unnormalised inverse transforms carry a factor N = n^3.
+//
+// Backward, recomputing the per-radius fields from the forward spectrum Z:
+//   sweep 1 (per radius batch): W_R Z -> inverse -> (N R_R, s N I_R) ->
+//     phi_R, two radii packed per complex line (phi_a + i phi_b) ->
+//     forward -> C += (B_a - i B_b) FFT(phi_a + i phi_b); Re IFFT(C)/N = A.
+//     (B_R real and even: Re IFFT((B_a - i B_b) FFT(phi_a + i phi_b)) =
+//      N (B_a (*) phi_a + B_b (*) phi_b).)
+//   observables and their adjoints (Abar, qbar, direct Deltabar);
+//   sweep 2 (per radius batch): IFFT((B_a + i B_b) FFT(Abar)) = N (phibar_a
+//     + i phibar_b); recomputed pair -> seeds (Rbar_R + i Ibar_R) -> forward
+//     -> G += W_R FFT(seed_R); IFFT(G)/N = recbar + i ionbar.
+#include "fft_core.h"
+#include "kernels.h"
+
+namespace sf21 {
+
+static unsigned eb(long count) {
+  long b = (count + 255) / 256;
+  if (b > 4096) b = 4096;
+  if (b < 1) b = 1;
+  return (unsigned)b;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 block reductions
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void block_reduce2(double a, double b, double* r1, double* r2) {
+  r1[threadIdx.x] = a;
+  r2[threadIdx.x] = b;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      r1[threadIdx.x] += r1[threadIdx.x + s];
+      r2[threadIdx.x] += r2[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+}
+
+// out[slot0] = sum part[2i], out[slot1] = sum part[2i+1]
+__global__ __launch_bounds__(256) void k_final2(const double* __restrict__ part, int np,
+                                                double* __restrict__ out, int slot0, int slot1) {
+  __shared__ double r1[256], r2[256];
+  double a = 0.0, b = 0.0;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) {
+    a += part[2 * i];
+    b += part[2 * i + 1];
+  }
+  block_reduce2(a, b, r1, r2);
+  if (threadIdx.x == 0) {
+    if (slot0 >= 0) out[slot0] = r1[0];
+    if (slot1 >= 0) out[slot1] = r2[0];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_sum_exp(long n3, const float* __restrict__ g,
+                                                 double* __restrict__ part) {
+  __shared__ double r1[256], r2[256];
+  double s = 0.0;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n3;
+       i += (long)gridDim.x * blockDim.x)
+    s += (double)expf(g[i]);
+  block_reduce2(s, 0.0, r1, r2);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = r1[0];
+    part[2 * blockIdx.x + 1] = 0.0;
+  }
+}
+
+// Delta = exp(g) N / sum exp(g), rec = Delta^2, ion = zeta E; sum(rec) partials.
+__global__ __launch_bounds__(256) void k_make_fields(long n3, const float* __restrict__ g,
+                                                     const float* __restrict__ emis, float zf,
+                                                     const double* __restrict__ st,
+                                                     float* __restrict__ delta1,
+                                                     float* __restrict__ rec,
+                                                     float* __restrict__ ion,
+                                                     double* __restrict__ part) {
+  __shared__ double r1[256], r2[256];
+  const float inv_mu = (float)((double)n3 / st[DS_SUM_EXP]);
+  double sr = 0.0;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n3;
+       i += (long)gridDim.x * blockDim.x) {
+    const float d = expf(g[i]) * inv_mu;
+    delta1[i] = d;
+    const float r = d * d;
+    rec[i] = r;
+    ion[i] = zf * emis[i];
+    sr += (double)r;
+  }
+  block_reduce2(sr, 0.0, r1, r2);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = r1[0];
+    part[2 * blockIdx.x + 1] = 0.0;
+  }
+}
+
+// T = (1 - x) Delta (exact forward), sum(x) partials.
+__global__ __launch_bounds__(256) void k_t_field(long n3, const float* __restrict__ x,
+                                                 const float* __restrict__ delta1,
+                                                 float* __restrict__ t,
+                                                 double* __restrict__ part) {
+  __shared__ double r1[256], r2[256];
+  double sx = 0.0;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n3;
+       i += (long)gridDim.x * blockDim.x) {
+    const float xv = x[i];
+    t[i] = (1.0f - xv) * delta1[i];
+    sx += (double)xv;
+  }
+  block_reduce2(sx, 0.0, r1, r2);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = r1[0];
+    part[2 * blockIdx.x + 1] = 0.0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Ball spectra: B_R = FFT of the indicator of {d : fold(d_i)^2 + fold(d_j)^2
+// + fold(d_p)^2 <= d2max(R)} (the exact ball of the dilation, each periodic
+// cell once); real and even in each axis, kept folded as [fi][fj][fp] with
+// f in [0, n/2].
+// ---------------------------------------------------------------------------
+__global__ void k_ball_box(int n, int d2, float2* __restrict__ out) {
+  const long n3 = (long)n * n * n;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n3;
+       i += (long)gridDim.x * blockDim.x) {
+    const int p = (int)(i % n), j = (int)((i / n) % n), a = (int)(i / ((long)n * n));
+    const int m = fold_sq(a, n) + fold_sq(j, n) + fold_sq(p, n);
+    out[i] = make_float2(m <= d2 ? 1.0f : 0.0f, 0.0f);
+  }
+}
+
+__global__ void k_ball_fold(int n, const float2* __restrict__ spec, float* __restrict__ table) {
+  const int H = n / 2 + 1;
+  const long h3 = (long)H * H * H;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < h3;
+       i += (long)gridDim.x * blockDim.x) {
+    const int fp = (int)(i % H), fj = (int)((i / H) % H), fa = (int)(i / ((long)H * H));
+    table[i] = spec[((long)fa * n + fj) * n + fp].x;
+  }
+}
+
+__device__ __forceinline__ int fold_idx(int i, int n) { return i > (n >> 1) ? n - i : i; }
+
+// Returns v unchanged but opaque to the optimiser.  Used on the lane index
+// inside radius loops: without it the compiler hoists every stage's LDS and
+// twiddle addresses out of the loop and keeps them live in VGPRs (spilling
+// at 1024-thread blocks); recomputing them costs a few VALU per iteration.
+__device__ __forceinline__ int opaque(int v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// Relaxation helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float softplus_f(float v) {
+  return v > 20.0f ? v : (v < -30.0f ? expf(v) : log1pf(expf(v)));
+}
+__device__ __forceinline__ float sigmoid_f(float v) { return 1.0f / (1.0f + expf(-v)); }
+
+// rho in unnormalised units: N rho = 1e-3 sum(rec)
+__device__ __forceinline__ float rho_u_of(const double* st) {
+  return (float)(1e-3 * st[DS_SUM_REC]);
+}
+
+// ---------------------------------------------------------------------------
+// Sweep 1, rows: inverse along p of each radius' pair, phi_R, pairs packed
+// and transformed forward along p, written in place to slot r/2 (slot r/2's
+// row of this block was read at an earlier or the same iteration).
+// ---------------------------------------------------------------------------
+template <int L>
+__global__ __launch_bounds__(256) void k_rows_phi(float2* u, int nrad, long ustride,
+                                                  const float* __restrict__ bal,
+                                                  const double* __restrict__ st, float inv_tau,
+                                                  const float2* __restrict__ tw) {
+  constexpr int E = Plan<L>::E, TPL = L / E, LPB = 256 / TPL;
+  __shared__ float lds[LPB * 2 * LdsLine<L>::kFloats];
+  const int lw = threadIdx.x / TPL, t = threadIdx.x % TPL;
+  const long row = (long)blockIdx.x * LPB + lw;
+  float* sre = lds + lw * 2 * LdsLine<L>::kFloats;
+  float* sim = sre + LdsLine<L>::kFloats;
+  const float inv_s = 1.0f / bal[0];
+  const float rho_u = rho_u_of(st);
+  float pa[E];
+#pragma unroll
+  for (int m = 0; m < E; ++m) pa[m] = 0.0f;
+#pragma unroll 1
+  for (int r = 0; r < nrad; ++r) {
+    const int tl = opaque(t);
+    float2 v[E];
+    const float2* src = u + (long)r * ustride + row * L;
+#pragma unroll
+    for (int m = 0; m < E; ++m) v[m] = src[tl + TPL * m];
+    line_fft<L, +1>(v, sre, sim, tl, tw);
+    float ph[E];
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const float ru = v[m].x, iu = v[m].y * inv_s;
+      ph[m] = softplus_f(fminf(iu - ru, iu) / fmaxf(ru, rho_u) * inv_tau);
+    }
+    if ((r & 1) == 0 && r + 1 < nrad) {
+#pragma unroll
+      for (int m = 0; m < E; ++m) pa[m] = ph[m];
+      continue;
+    }
+    float2 w[E];
+#pragma unroll
+    for (int m = 0; m < E; ++m) w[m] = (r & 1) ? make_float2(pa[m], ph[m]) : make_float2(ph[m], 0.0f);
+    line_fft<L, -1>(w, sre, sim, tl, tw);
+    float2* dst = u + (long)(r >> 1) * ustride + row * L;
+#pragma unroll
+    for (int m = 0; m < E; ++m) dst[tl + TPL * m] = w[m];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Ball passes along i.  bt: folded ball tables of the batch's radii (table r
+// at bt + r*h3).  Sweep 1 (ACC): per pair, forward along i, times
+// (B_a - i B_b), summed over the batch's pairs, one inverse along i, added to
+// out ((p,j)-space accumulator).  Sweep 2 (!ACC): the (p,j)-space Abar line
+// forward along i once, per pair times (B_a + i B_b), inverse along i, to
+// slot pr of out.
+// ---------------------------------------------------------------------------
+template <int L, int TILE, bool ACC>
+__global__ __launch_bounds__(TILE*(L / Plan<L>::E)) void k_ball_i(
+    const float2* in, long istride, float2* out, long ostride, int nrad,
+    const float* __restrict__ bt, long h3, const float2* __restrict__ tw) {
+  constexpr int E = Plan<L>::E, TPL = L / E, NT = L / TILE, H = L / 2 + 1;
+  __shared__ float lds[TILE * 2 * LdsLine<L>::kFloats];
+  const int c = threadIdx.x % TILE, t = threadIdx.x / TILE;
+  const int kj = blockIdx.x / NT;
+  const int kp = (blockIdx.x % NT) * TILE + c;
+  const long off = (long)kj * L + kp;
+  float* sre = lds + c * 2 * LdsLine<L>::kFloats;
+  float* sim = sre + LdsLine<L>::kFloats;
+  const int bjp = fold_idx(kj, L) * H + fold_idx(kp, L);
+  const int npair = (nrad + 1) / 2;
+  float2 z[E];
+  if constexpr (ACC) {
+#pragma unroll
+    for (int m = 0; m < E; ++m) z[m] = make_float2(0.f, 0.f);
+  } else {
+#pragma unroll
+    for (int m = 0; m < E; ++m) z[m] = in[off + (t + TPL * m) * (L * L)];
+    line_fft<L, -1>(z, sre, sim, t, tw);
+  }
+#pragma unroll 1
+  for (int pr = 0; pr < npair; ++pr) {
+    const int tl = opaque(t);
+    const float* __restrict__ ta = bt + (long)(2 * pr) * h3;
+    const float* __restrict__ tb = bt + (long)(2 * pr + 1) * h3;
+    const bool two = 2 * pr + 1 < nrad;
+    float2 v[E];
+    if constexpr (ACC) {
+      const float2* src = in + (long)pr * istride + off;
+#pragma unroll
+      for (int m = 0; m < E; ++m) v[m] = src[(tl + TPL * m) * (L * L)];
+      line_fft<L, -1>(v, sre, sim, tl, tw);
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const int ti = fold_idx(tl + TPL * m, L) * (H * H) + bjp;
+        const float ba = ta[ti], bb = two ? tb[ti] : 0.0f;
+        z[m].x += ba * v[m].x + bb * v[m].y;
+        z[m].y += ba * v[m].y - bb * v[m].x;
+      }
+    } else {
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const int ti = fold_idx(tl + TPL * m, L) * (H * H) + bjp;
+        const float ba = ta[ti], bb = two ? tb[ti] : 0.0f;
+        v[m] = make_float2(ba * z[m].x - bb * z[m].y, ba * z[m].y + bb * z[m].x);
+      }
+      line_fft<L, +1>(v, sre, sim, tl, tw);
+      float2* dst = out + (long)pr * ostride + off;
+#pragma unroll
+      for (int m = 0; m < E; ++m) dst[(tl + TPL * m) * (L * L)] = v[m];
+    }
+  }
+  if constexpr (ACC) {
+    line_fft<L, +1>(z, sre, sim, t, tw);
+    float2* dst = out + off;
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const float2 o = dst[(t + TPL * m) * (L * L)];
+      dst[(t + TPL * m) * (L * L)] = make_float2(o.x + z[m].x, o.y + z[m].y);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Sweep 2, rows: inverse along p of the recomputed pair (N R_R, s N I_R) and
+// of the ball pair (N phibar_a, N phibar_b); seeds of the normalised fields
+//   sbar = phibar sigmoid(s/tau)/tau,
+//   Rbar = sbar ds/dR,  Ibar = sbar ds/dI,
+//   ds/dI = 1/den, ds/dR = -[R >= 0]/den - [R > rho] num/den^2,
+//   num = min(I - R, I), den = max(R, rho)
+// packed Rbar + i Ibar, forward along p, in place over the pair.  The rho
+// term sbar ds/drho = -sbar [R <= rho] num/rho^2 is summed in FP64 into
+// rho_acc[block] (accumulated over the batches).  In unnormalised units the
+// factors N of ds/d(.) and of phibar cancel.
+// ---------------------------------------------------------------------------
+template <int L>
+__global__ __launch_bounds__(256) void k_rows_seed(float2* u, const float2* v, int nrad,
+                                                   long ustride, long vstride,
+                                                   const float* __restrict__ bal,
+                                                   const double* __restrict__ st, float inv_tau,
+                                                   double* __restrict__ rho_acc,
+                                                   const float2* __restrict__ tw) {
+  constexpr int E = Plan<L>::E, TPL = L / E, LPB = 256 / TPL;
+  __shared__ float lds[LPB * 2 * LdsLine<L>::kFloats];
+  __shared__ double r1[256], r2[256];
+  const int lw = threadIdx.x / TPL, t = threadIdx.x % TPL;
+  const long row = (long)blockIdx.x * LPB + lw;
+  float* sre = lds + lw * 2 * LdsLine<L>::kFloats;
+  float* sim = sre + LdsLine<L>::kFloats;
+  const float inv_s = 1.0f / bal[0];
+  const float rho_u = rho_u_of(st);
+  double rsum = 0.0;
+  const int npair = (nrad + 1) / 2;
+#pragma unroll 1
+  for (int pr = 0; pr < npair; ++pr) {
+    float2 pb[E];
+    {
+      const int tl = opaque(t);
+      const float2* bs = v + (long)pr * vstride + row * L;
+#pragma unroll
+      for (int m = 0; m < E; ++m) pb[m] = bs[tl + TPL * m];
+      line_fft<L, +1>(pb, sre, sim, tl, tw);
+    }
+    const int nh = (2 * pr + 1 < nrad) ? 2 : 1;
+#pragma unroll 1
+    for (int hh = 0; hh < nh; ++hh) {
+      const int tl = opaque(t);
+      float2 y[E];
+      float2* src = u + (long)(2 * pr + hh) * ustride + row * L;
+#pragma unroll
+      for (int m = 0; m < E; ++m) y[m] = src[tl + TPL * m];
+      line_fft<L, +1>(y, sre, sim, tl, tw);
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const float ru = y[m].x, iu = y[m].y * inv_s;
+        const float na = iu - ru;
+        const bool use_a = na <= iu;
+        const float num = use_a ? na : iu;
+        const bool den_r = ru > rho_u;
+        const float den = den_r ? ru : rho_u;
+        const float inv_den = 1.0f / den;
+        const float s = num * inv_den;
+        const float phib = hh ? pb[m].y : pb[m].x;
+        const float sb = phib * sigmoid_f(s * inv_tau) * inv_tau;
+        const float q = num * inv_den * inv_den;
+        const float dR = (use_a ? -inv_den : 0.0f) - (den_r ? q : 0.0f);
+        y[m] = make_float2(sb * dR, sb * inv_den);
+        if (!den_r) rsum -= (double)sb * (double)q;
+      }
+      line_fft<L, -1>(y, sre, sim, tl, tw);
+#pragma unroll
+      for (int m = 0; m < E; ++m) src[tl + TPL * m] = y[m];
+    }
+  }
+  block_reduce2(rsum, 0.0, r1, r2);
+  if (threadIdx.x == 0) rho_acc[blockIdx.x] += r1[0];
+}
+
+// ---------------------------------------------------------------------------
+// Sweep 2, along i: forward of each seed line, times W_R, summed over the
+// batch, one inverse along i, added to acc ((p,j)-space).
+// ---------------------------------------------------------------------------
+template <int L, int TILE>
+__global__ __launch_bounds__(TILE*(L / Plan<L>::E)) void k_filter_acc_i(
+    const float2* in, long istride, int nrad, const float* __restrict__ wlut, long mlut,
+    float2* acc, const float2* __restrict__ tw) {
+  constexpr int E = Plan<L>::E, TPL = L / E, NT = L / TILE;
+  __shared__ float lds[TILE * 2 * LdsLine<L>::kFloats];
+  const int c = threadIdx.x % TILE, t = threadIdx.x / TILE;
+  const int kj = blockIdx.x / NT;
+  const int kp = (blockIdx.x % NT) * TILE + c;
+  const long off = (long)kj * L + kp;
+  float* sre = lds + c * 2 * LdsLine<L>::kFloats;
+  float* sim = sre + LdsLine<L>::kFloats;
+  const int mjp = fold_sq(kj, L) + fold_sq(kp, L);
+  float2 z[E];
+#pragma unroll
+  for (int m = 0; m < E; ++m) z[m] = make_float2(0.f, 0.f);
+#pragma unroll 1
+  for (int r = 0; r < nrad; ++r) {
+    const int tl = opaque(t);
+    const float2* src = in + (long)r * istride + off;
+    float2 v[E];
+#pragma unroll
+    for (int m = 0; m < E; ++m) v[m] = src[(tl + TPL * m) * (L * L)];
+    line_fft<L, -1>(v, sre, sim, tl, tw);
+    const float* __restrict__ w = wlut + (long)r * mlut;
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const float wv = w[mjp + fold_sq(tl + TPL * m, L)];
+      z[m].x += wv * v[m].x;
+      z[m].y += wv * v[m].y;
+    }
+  }
+  line_fft<L, +1>(z, sre, sim, t, tw);
+  float2* dst = acc + off;
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const float2 o = dst[(t + TPL * m) * (L * L)];
+    dst[(t + TPL * m) * (L * L)] = make_float2(o.x + z[m].x, o.y + z[m].y);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Elementwise passes
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float quick_fill_d(float rec, float ion) {
+  if (ion > 0.0f) return rec > 0.0f ? fminf(ion / rec, 1.0f) : 1.0f;
+  return 0.0f;
+}
+
+// From C = N A + i(...): x~, T = (1 - x~) Delta, dx~/dq = exp(-A),
+// dx~/dA = (1 - q) exp(-A); sum(x~) partials.
+__global__ __launch_bounds__(256) void k_relaxed_state(long n3, const float2* __restrict__ cacc,
+                                                       const float* __restrict__ rec,
+                                                       const float* __restrict__ ion,
+                                                       const float* __restrict__ delta1,
+                                                       float* __restrict__ xt,
+                                                       float* __restrict__ t,
+                                                       float* __restrict__ fq,
+                                                       float* __restrict__ fa,
+                                                       double* __restrict__ part) {
+  __shared__ double r1[256], r2[256];
+  const float inv_n3 = (float)(1.0 / (double)n3);
+  double sx = 0.0;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n3;
+       i += (long)gridDim.x * blockDim.x) {
+    const float A = cacc[i].x * inv_n3;
+    const float e = expf(-A);
+    const float q = quick_fill_d(rec[i], ion[i]);
+    const float x = q + (1.0f - q) * (1.0f - e);
+    xt[i] = x;
+    t[i] = (1.0f - x) * delta1[i];
+    fq[i] = e;
+    fa[i] = (1.0f - q) * e;
+    sx += (double)x;
+  }
+  block_reduce2(sx, 0.0, r1, r2);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = r1[0];
+    part[2 * blockIdx.x + 1] = 0.0;
+  }
+}
+
+// P(k) adjoint on the half spectrum F of T: H = c_b F with c_b = dL/dP_b *
+// norm / count_b, doubled on the planes p = 0 and p = n/2, so that the
+// unnormalised C2R of H is dL/dT.
+__global__ void k_pk_adjoint(int n, float2* __restrict__ spec, const short* __restrict__ bin_of_m,
+                             const double* __restrict__ coef) {
+  const int h = n / 2 + 1;
+  const long nh = (long)n * n * h;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < nh;
+       i += (long)gridDim.x * blockDim.x) {
+    const int q = (int)(i % h);
+    const long r = i / h;
+    const int m = fold_sq((int)(r / n), n) + fold_sq((int)(r % n), n) + q * q;
+    const int b = bin_of_m[m];
+    float c = b >= 0 ? (float)coef[b] : 0.0f;
+    if (q == 0 || q == n / 2) c *= 2.0f;
+    const float2 v = spec[i];
+    spec[i] = make_float2(v.x * c, v.y * c);
+  }
+}
+
+// Adjoint coefficients of the binned P(k): c_b = dL/dP_b norm / count_b
+// (counts from the P(k) pass just run, pk[2 nbins + b]).
+__global__ void k_pk_coef(const double* __restrict__ gp, const double* __restrict__ pk, int nbins,
+                          double norm, double* __restrict__ coef) {
+  const int b = threadIdx.x;
+  if (b >= nbins) return;
+  const double cnt = pk[2 * nbins + b];
+  coef[b] = cnt > 0 ? gp[b] * norm / cnt : 0.0;
+}
+
+// xbar = gx - Tbar Delta (gx = dL/dX / N);  Abar = xbar (1 - q) exp(-A) (as a complex
+// box);  qbar = xbar exp(-A);  direct Deltabar = Tbar (1 - x~).
+__global__ void k_seed_a(long n3, const float* __restrict__ tbar, const float* __restrict__ delta1,
+                         const float* __restrict__ xt, const float* __restrict__ fq,
+                         const float* __restrict__ fa, float gx,
+                         float2* __restrict__ abar, float* __restrict__ qbar,
+                         float* __restrict__ dbar) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n3;
+       i += (long)gridDim.x * blockDim.x) {
+    const float tb = tbar[i];
+    const float xb = gx - tb * delta1[i];
+    abar[i] = make_float2(xb * fa[i], 0.0f);
+    qbar[i] = xb * fq[i];
+    dbar[i] = tb * (1.0f - xt[i]);
+  }
+}
+
+// Per cell: recbar, ionbar from G = N (recbar + i ionbar) plus the quick-fill
+// and rho terms, Deltabar = direct + 2 Delta recbar; FP64 partials of
+// sum(ionbar E) (-> dL/dzeta) and sum(Deltabar Delta).
+__global__ __launch_bounds__(256) void k_grad_cells(long n3, const float2* __restrict__ gacc,
+                                                    const float* __restrict__ qbar,
+                                                    const float* __restrict__ rec,
+                                                    const float* __restrict__ ion,
+                                                    const float* __restrict__ emis,
+                                                    const float* __restrict__ delta1,
+                                                    const float* __restrict__ dbar,
+                                                    const double* __restrict__ st,
+                                                    float* __restrict__ dd_out,
+                                                    double* __restrict__ part) {
+  __shared__ double r1[256], r2[256];
+  const float inv_n3 = (float)(1.0 / (double)n3);
+  // rho = 1e-3 sum(rec)/N: drho/drec = 1e-3/N per cell
+  const float rho_coef = (float)(st[DS_RHO_BAR] * 1e-3 / (double)n3);
+  double sz = 0.0, sd = 0.0;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n3;
+       i += (long)gridDim.x * blockDim.x) {
+    const float2 gs = gacc[i];
+    float rb = gs.x * inv_n3 + rho_coef;
+    float ib = gs.y * inv_n3;
+    const float r = rec[i], o = ion[i];
+    if (o > 0.0f && r > 0.0f && o < r) {
+      const float qb = qbar[i];
+      ib += qb / r;
+      rb -= qb * o / (r * r);
+    }
+    const float d = delta1[i];
+    const float ddel = dbar[i] + 2.0f * d * rb;
+    dd_out[i] = ddel;
+    sz += (double)ib * (double)emis[i];
+    sd += (double)ddel * (double)d;
+  }
+  block_reduce2(sz, sd, r1, r2);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = r1[0];
+    part[2 * blockIdx.x + 1] = r2[0];
+  }
+}
+
+// gbar = Delta (Deltabar - S/N), S = sum(Deltabar Delta)  (Delta = e^g/mean e^g)
+__global__ void k_grad_g(long n3, const float* __restrict__ delta1, const float* __restrict__ dd,
+                         const double* __restrict__ st, float* __restrict__ gout) {
+  const float sbar = (float)(st[DS_S] / (double)n3);
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n3;
+       i += (long)gridDim.x * blockDim.x)
+    gout[i] = delta1[i] * (dd[i] - sbar);
+}
+
+__global__ void k_fill_c(float2* __restrict__ p, long count) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+       i += (long)gridDim.x * blockDim.x)
+    p[i] = make_float2(0.f, 0.f);
+}
+__global__ void k_fill_d(double* __restrict__ p, long count) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+       i += (long)gridDim.x * blockDim.x)
+    p[i] = 0.0;
+}
+__global__ void k_to_complex(long n3, const float* __restrict__ in, float2* __restrict__ out) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n3;
+       i += (long)gridDim.x * blockDim.x)
+    out[i] = make_float2(in[i], 0.0f);
+}
+
+// sum of the per-block rho accumulators -> st[DS_RHO_BAR]
+__global__ __launch_bounds__(256) void k_sum_rho(const double* __restrict__ acc, int np,
+                                                 double* __restrict__ st) {
+  __shared__ double r1[256], r2[256];
+  double a = 0.0;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) a += acc[i];
+  block_reduce2(a, 0.0, r1, r2);
+  if (threadIdx.x == 0) st[DS_RHO_BAR] = r1[0];
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+int diff_partials_count(long n3) { return (int)eb(n3); }
+
+void launch_sum_exp(long n3, const float* g, double* part, double* st, hipStream_t s) {
+  hipLaunchKernelGGL(k_sum_exp, dim3(eb(n3)), dim3(256), 0, s, n3, g, part);
+  hipLaunchKernelGGL(k_final2, dim3(1), dim3(256), 0, s, part, (int)eb(n3), st, DS_SUM_EXP, -1);
+}
+void launch_make_fields(long n3, const float* g, const float* emis, float zeta, double* st,
+                        float* delta1, float* rec, float* ion, double* part, hipStream_t s) {
+  hipLaunchKernelGGL(k_make_fields, dim3(eb(n3)), dim3(256), 0, s, n3, g, emis, zeta, st, delta1,
+                     rec, ion, part);
+  hipLaunchKernelGGL(k_final2, dim3(1), dim3(256), 0, s, part, (int)eb(n3), st, DS_SUM_REC, -1);
+}
+void launch_t_field(long n3, const float* x, const float* delta1, float* t, double* part,
+                    double* st, hipStream_t s) {
+  hipLaunchKernelGGL(k_t_field, dim3(eb(n3)), dim3(256), 0, s, n3, x, delta1, t, part);
+  hipLaunchKernelGGL(k_final2, dim3(1), dim3(256), 0, s, part, (int)eb(n3), st, DS_SUM_X, -1);
+}
+void launch_ball_box(int n, int d2, float2* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_ball_box, dim3(eb((long)n * n * n)), dim3(256), 0, s, n, d2, out);
+}
+void launch_ball_fold(int n, const float2* spec, float* table, hipStream_t s) {
+  const long h = n / 2 + 1;
+  hipLaunchKernelGGL(k_ball_fold, dim3(eb(h * h * h)), dim3(256), 0, s, n, spec, table);
+}
+
+template <int L>
+static void rows_phi_t(int n, float2* u, int nrad, long ustride, const float* bal, const double* st,
+                       float inv_tau, const float2* tw, hipStream_t s) {
+  constexpr int LPB = 256 / (L / Plan<L>::E);
+  hipLaunchKernelGGL(k_rows_phi<L>, dim3((unsigned)((long)n * n / LPB)), dim3(256), 0, s, u, nrad,
+                     ustride, bal, st, inv_tau, tw);
+}
+template <int L>
+static void rows_seed_t(int n, float2* u, const float2* v, int nrad, long ustride, long vstride,
+                        const float* bal, const double* st, float inv_tau, double* rho_acc,
+                        const float2* tw, hipStream_t s) {
+  constexpr int LPB = 256 / (L / Plan<L>::E);
+  hipLaunchKernelGGL(k_rows_seed<L>, dim3((unsigned)((long)n * n / LPB)), dim3(256), 0, s, u, v,
+                     nrad, ustride, vstride, bal, st, inv_tau, rho_acc, tw);
+}
+template <int L, bool ACC>
+static void ball_i_t(const float2* in, long istride, float2* out, long ostride, int nrad,
+                     const float* bt, const float2* tw, hipStream_t s) {
+  constexpr int TILE = 16, TPL = L / Plan<L>::E;
+  const long H = L / 2 + 1;
+  hipLaunchKernelGGL((k_ball_i<L, TILE, ACC>), dim3((unsigned)((long)L * (L / TILE))),
+                     dim3(TILE * TPL), 0, s, in, istride, out, ostride, nrad, bt, H * H * H, tw);
+}
+template <int L>
+static void filter_acc_t(const float2* in, long istride, int nrad, const float* wlut, long mlut,
+                         float2* acc, const float2* tw, hipStream_t s) {
+  constexpr int TILE = 16, TPL = L / Plan<L>::E;
+  hipLaunchKernelGGL((k_filter_acc_i<L, TILE>), dim3((unsigned)((long)L * (L / TILE))),
+                     dim3(TILE * TPL), 0, s, in, istride, nrad, wlut, mlut, acc, tw);
+}
+
+#define SF21_DIFF_SIZES(X) X(32) X(64) X(96) X(128) X(192) X(256) X(512)
+
+bool diff_size_supported(int n) {
+  switch (n) {
+#define C(L) case L: return true;
+    SF21_DIFF_SIZES(C)
+#undef C
+  }
+  return false;
+}
+int rows_blocks(int n) {
+  switch (n) {
+#define C(L) case L: return (int)((long)n * n / (256 / (L / Plan<L>::E)));
+    SF21_DIFF_SIZES(C)
+#undef C
+  }
+  return 0;
+}
+void launch_rows_phi(int n, float2* u, int nrad, long ustride, const float* bal, const double* st,
+                     float inv_tau, const float2* tw, hipStream_t s) {
+  switch (n) {
+#define C(L) case L: rows_phi_t<L>(n, u, nrad, ustride, bal, st, inv_tau, tw, s); break;
+    SF21_DIFF_SIZES(C)
+#undef C
+  }
+}
+void launch_rows_seed(int n, float2* u, const float2* v, int nrad, long ustride, long vstride,
+                      const float* bal, const double* st, float inv_tau, double* rho_acc,
+                      const float2* tw, hipStream_t s) {
+  switch (n) {
+#define C(L) case L: rows_seed_t<L>(n, u, v, nrad, ustride, vstride, bal, st, inv_tau, rho_acc, tw, s); break;
+    SF21_DIFF_SIZES(C)
+#undef C
+  }
+}
+void launch_ball_i(int n, bool acc, const float2* in, long istride, float2* out, long ostride,
+                   int nrad, const float* bt, const float2* tw, hipStream_t s) {
+  switch (n) {
+#define C(L)                                                                   \
+  case L:                                                                      \
+    if (acc) ball_i_t<L, true>(in, istride, out, ostride, nrad, bt, tw, s);    \
+    else ball_i_t<L, false>(in, istride, out, ostride, nrad, bt, tw, s);       \
+    break;
+    SF21_DIFF_SIZES(C)
+#undef C
+  }
+}
+void launch_filter_acc_i(int n, const float2* in, long istride, int nrad, const float* wlut,
+                         long mlut, float2* acc, const float2* tw, hipStream_t s) {
+  switch (n) {
+#define C(L) case L: filter_acc_t<L>(in, istride, nrad, wlut, mlut, acc, tw, s); break;
+    SF21_DIFF_SIZES(C)
+#undef C
+  }
+}
+
+void launch_relaxed_state(long n3, const float2* cacc, const float* rec, const float* ion,
+                          const float* delta1, float* xt, float* t, float* fq, float* fa,
+                          double* part, double* st, hipStream_t s) {
+  hipLaunchKernelGGL(k_relaxed_state, dim3(eb(n3)), dim3(256), 0, s, n3, cacc, rec, ion, delta1, xt,
+                     t, fq, fa, part);
+  hipLaunchKernelGGL(k_final2, dim3(1), dim3(256), 0, s, part, (int)eb(n3), st, DS_SUM_XT, -1);
+}
+void launch_pk_adjoint(int n, float2* spec, const short* bin_of_m, const double* coef,
+                       hipStream_t s) {
+  const long nh = (long)n * n * (n / 2 + 1);
+  hipLaunchKernelGGL(k_pk_adjoint, dim3(eb(nh)), dim3(256), 0, s, n, spec, bin_of_m, coef);
+}
+void launch_pk_coef(const double* gp, const double* pk, int nbins, double norm, double* coef,
+                    hipStream_t s) {
+  hipLaunchKernelGGL(k_pk_coef, dim3(1), dim3(64), 0, s, gp, pk, nbins, norm, coef);
+}
+void launch_seed_a(long n3, const float* tbar, const float* delta1, const float* xt,
+                   const float* fq, const float* fa, float gx_n, float2* abar, float* qbar,
+                   float* dbar, hipStream_t s) {
+  hipLaunchKernelGGL(k_seed_a, dim3(eb(n3)), dim3(256), 0, s, n3, tbar, delta1, xt, fq, fa, gx_n,
+                     abar, qbar, dbar);
+}
+void launch_sum_rho(const double* acc, int np, double* st, hipStream_t s) {
+  hipLaunchKernelGGL(k_sum_rho, dim3(1), dim3(256), 0, s, acc, np, st);
+}
+void launch_grad_cells(long n3, const float2* gacc, const float* qbar, const float* rec,
+                       const float* ion, const float* emis, const float* delta1,
+                       const float* dbar, double* st, float* dd_out, double* part,
+                       hipStream_t s) {
+  hipLaunchKernelGGL(k_grad_cells, dim3(eb(n3)), dim3(256), 0, s, n3, gacc, qbar, rec, ion, emis,
+                     delta1, dbar, st, dd_out, part);
+  hipLaunchKernelGGL(k_final2, dim3(1), dim3(256), 0, s, part, (int)eb(n3), st, DS_ZETA_BAR, DS_S);
+}
+void launch_grad_g(long n3, const float* delta1, const float* dd, const double* st, float* gout,
+                   hipStream_t s) {
+  hipLaunchKernelGGL(k_grad_g, dim3(eb(n3)), dim3(256), 0, s, n3, delta1, dd, st, gout);
+}
+void launch_fill_c(float2* p, long count, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_c, dim3(eb(count)), dim3(256), 0, s, p, count);
+}
+void launch_fill_d(double* p, long count, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_d, dim3(eb(count)), dim3(256), 0, s, p, count);
+}
+void launch_to_complex(long n3, const float* in, float2* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_to_complex, dim3(eb(n3)), dim3(256), 0, s, n3, in, out);
+}
+
+}  // namespace sf21

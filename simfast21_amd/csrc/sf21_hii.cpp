@@ -109,6 +109,16 @@ struct sf21_hii_ctx {
   double* pk_part = nullptr;
   double* pk_out = nullptr;
   int pk_nbins = 0;
+  // differentiable mode (allocated on first use)
+  float *df_delta = nullptr, *df_rec = nullptr, *df_ion = nullptr, *df_x = nullptr;
+  float *df_t = nullptr, *df_fq = nullptr, *df_fa = nullptr, *df_qbar = nullptr;
+  float *df_dbar = nullptr, *df_dd = nullptr;
+  float2 *df_v = nullptr, *df_c = nullptr, *df_g = nullptr, *df_half = nullptr;
+  float* df_ball = nullptr;  // folded ball spectra, (n/2+1)^3 per radius
+  double *df_st = nullptr, *df_part = nullptr, *df_rho = nullptr, *df_gp = nullptr;
+  double* df_coef = nullptr;
+  double* df_host = nullptr;  // pinned staging
+  bool df_ready = false;
   // halo mode
   float* halo_grid = nullptr;   // N_halo^3 deposit grid
   float* halo_buf = nullptr;    // catalog upload (host entry point)
@@ -359,6 +369,14 @@ void free_all(sf21_hii_ctx* c) {
   dev_free(c->spec2); dev_free(c->tmp2); dev_free(c->th); dev_free(c->flag_dev);
   dev_free(c->halo_grid); dev_free(c->halo_buf);
   dev_free(c->pk_bin); dev_free(c->pk_kabs); dev_free(c->pk_part); dev_free(c->pk_out);
+  dev_free(c->df_delta); dev_free(c->df_rec); dev_free(c->df_ion); dev_free(c->df_x);
+  dev_free(c->df_t); dev_free(c->df_fq); dev_free(c->df_fa); dev_free(c->df_qbar);
+  dev_free(c->df_dbar); dev_free(c->df_dd); dev_free(c->df_v); dev_free(c->df_c);
+  dev_free(c->df_g); dev_free(c->df_half); dev_free(c->df_ball); dev_free(c->df_st);
+  dev_free(c->df_part); dev_free(c->df_rho); dev_free(c->df_gp); dev_free(c->df_coef);
+  if (c->df_host) (void)hipHostFree(c->df_host);
+  c->df_host = nullptr;
+  c->df_ready = false;
   if (c->flag_host) (void)hipHostFree(c->flag_host);
   if (c->neutral_host) (void)hipHostFree(c->neutral_host);
   c->flag_host = nullptr;
@@ -367,6 +385,213 @@ void free_all(sf21_hii_ctx* c) {
   c->pool.clear();
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
   c->stream = nullptr;
+}
+
+// ---- binned power spectrum: tables and launch ---------------------------
+// Bin of every integer |k|^2 (in dk units): linear bins of |k| over [1, n/2],
+// width (n/2 - 1)/nbins, the last closed; k = 0 and the corners beyond n/2
+// are dropped (-1).
+int ensure_pk(sf21_hii_ctx* c, int nbins) {
+  if (c->pk_nbins == nbins) return SF21_OK;
+  const int hh = c->n / 2;
+  const long mmax = 3L * hh * hh;
+  std::vector<short> bins(mmax + 1);
+  std::vector<float> kab(mmax + 1);
+  const double width = (hh - 1) / (double)nbins;
+  for (long m = 0; m <= mmax; ++m) {
+    const double kq = std::sqrt((double)m);
+    int b = -1;
+    if (m >= 1 && kq <= hh) {
+      b = (int)std::floor((kq - 1.0) / width);
+      if (b >= nbins) b = nbins - 1;
+    }
+    bins[m] = (short)b;
+    kab[m] = (float)(kq * c->dk);
+  }
+  int rc;
+  dev_free(c->pk_bin); dev_free(c->pk_kabs); dev_free(c->pk_part); dev_free(c->pk_out);
+  c->pk_nbins = 0;
+  if ((rc = dev_alloc(&c->pk_bin, bins.size())) || (rc = dev_alloc(&c->pk_kabs, kab.size())) ||
+      (rc = dev_alloc(&c->pk_part, (size_t)3 * nbins * pk_partials_count(c->n))) ||
+      (rc = dev_alloc(&c->pk_out, (size_t)3 * nbins)))
+    return rc;
+  if (hipMemcpy(c->pk_bin, bins.data(), sizeof(short) * bins.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->pk_kabs, kab.data(), sizeof(float) * kab.size(), hipMemcpyHostToDevice) != hipSuccess)
+    return SF21_ERR_HIP;
+  c->pk_nbins = nbins;
+  return SF21_OK;
+}
+
+// P(k) of a real box (tools/power3d.c:172-191 normalisation, dl^3/N) into
+// c->pk_out: [0, nbins) power, [nbins, 2 nbins) mean |k|, then counts.
+// half: scratch of n*n*(n/2+1) complex values, left holding the spectrum.
+void enqueue_pk(sf21_hii_ctx* c, const float* field, float2* half, int nbins) {
+  r2c3d(c, field, half);
+  const double dl = c->dx;
+  const double norm = dl * dl * dl / ((double)c->n3);
+  launch(c, "power_spectrum", [&] {
+    launch_power_spectrum(c->n, half, c->pk_bin, c->pk_kabs, nbins, norm, c->pk_part, c->pk_out,
+                          c->stream);
+  });
+}
+
+// ---- differentiable mode ----------------------------------------------
+int ensure_diff(sf21_hii_ctx* c) {
+  if (c->df_ready) return SF21_OK;
+  const size_t n3 = (size_t)c->n3, nh = (size_t)c->nh;
+  const size_t h = (size_t)(c->n / 2 + 1);
+  const size_t npair = (size_t)(c->K + 1) / 2;
+  int rc;
+  if ((rc = dev_alloc(&c->df_delta, n3)) || (rc = dev_alloc(&c->df_rec, n3)) ||
+      (rc = dev_alloc(&c->df_ion, n3)) || (rc = dev_alloc(&c->df_x, n3)) ||
+      (rc = dev_alloc(&c->df_t, n3)) || (rc = dev_alloc(&c->df_fq, n3)) ||
+      (rc = dev_alloc(&c->df_fa, n3)) || (rc = dev_alloc(&c->df_qbar, n3)) ||
+      (rc = dev_alloc(&c->df_dbar, n3)) || (rc = dev_alloc(&c->df_dd, n3)) ||
+      (rc = dev_alloc(&c->df_v, npair * n3)) || (rc = dev_alloc(&c->df_c, n3)) ||
+      (rc = dev_alloc(&c->df_g, n3)) || (rc = dev_alloc(&c->df_half, nh)) ||
+      (rc = dev_alloc(&c->df_ball, h * h * h * c->radii.size())) ||
+      (rc = dev_alloc(&c->df_st, (size_t)DS_COUNT)) ||
+      (rc = dev_alloc(&c->df_part, (size_t)2 * diff_partials_count(c->n3))) ||
+      (rc = dev_alloc(&c->df_rho, (size_t)rows_blocks(c->n))) || (rc = dev_alloc(&c->df_gp, 64)) ||
+      (rc = dev_alloc(&c->df_coef, 64)))
+    return rc;
+  if (hipHostMalloc(reinterpret_cast<void**>(&c->df_host), sizeof(double) * 256) != hipSuccess) {
+    c->df_host = nullptr;
+    return SF21_ERR_ALLOC;
+  }
+  // ball spectra of every radius: FFT of the periodic ball indicator, folded
+  hipStream_t s = c->stream;
+  const int n = c->n;
+  for (size_t r = 0; r < c->radii.size(); ++r) {
+    launch_ball_box(n, c->d2[r], c->df_c, s);
+    launch_c2c_rows(n, -1, c->df_c, c->tw, s);
+    launch_cols(n, n, 0, -1, c->df_c, c->df_c, 1, 0, c->tw, s);
+    launch_cols(n, n, 1, -1, c->df_c, c->df_c, 1, 0, c->tw, s);
+    launch_ball_fold(n, c->df_c, c->df_ball + r * h * h * h, s);
+  }
+  if (hipStreamSynchronize(s) != hipSuccess) return SF21_ERR_HIP;
+  if ((rc = check_hip())) return rc;
+  c->df_ready = true;
+  return SF21_OK;
+}
+
+// Delta, rec, ion from (g, emis, zeta) and the FP64 sums they need.
+void enqueue_diff_fields(sf21_hii_ctx* c, const float* g, const float* emis, double zeta) {
+  hipStream_t s = c->stream;
+  launch(c, "diff_fields", [&] {
+    launch_sum_exp(c->n3, g, c->df_part, c->df_st, s);
+    launch_make_fields(c->n3, g, emis, (float)zeta, c->df_st, c->df_delta, c->df_rec, c->df_ion,
+                       c->df_part, s);
+  });
+}
+
+// Exact forward: x (bubble finder on rec = Delta^2, ion = zeta emis, direct
+// inputs), T = (1 - x) Delta, sum(x), P(k) of T.
+void enqueue_diff_forward(sf21_hii_ctx* c, const float* g, const float* emis, double zeta,
+                          int nbins, float* x) {
+  hipStream_t s = c->stream;
+  enqueue_diff_fields(c, g, emis, zeta);
+  enqueue_fused(c, c->df_rec, c->df_ion, nullptr, 0.0, x, c->neutral_dev, -1, nullptr);
+  launch(c, "diff_t", [&] { launch_t_field(c->n3, x, c->df_delta, c->df_t, c->df_part, c->df_st, s); });
+  enqueue_pk(c, c->df_t, c->df_half, nbins);
+}
+
+// Relaxed adjoint.  Expects df_gp (dL/dP_b) on the device.
+void enqueue_diff_backward(sf21_hii_ctx* c, const float* g, const float* emis, double zeta,
+                           double tau, int nbins, double gx, float* grad_g) {
+  hipStream_t s = c->stream;
+  const int n = c->n;
+  const long n3 = c->n3;
+  const int nr = (int)c->radii.size();
+  const long h = n / 2 + 1, h3 = h * h * h;
+  const float inv_tau = (float)(1.0 / tau);
+  const CellConsts k = cell_consts(c, 0.0);  // direct inputs
+  enqueue_diff_fields(c, g, emis, zeta);
+  // forward spectrum Z of rec + i s ion, (p,j)-space
+  launch(c, "balance", [&] { launch_balance(n3, c->df_rec, c->df_ion, nullptr, k, c->bal_part, c->bal, s); });
+  launch(c, "fwd_rows", [&] { launch_fwd_rows(n, c->df_rec, c->df_ion, nullptr, k, c->bal, c->spec, c->tw, s); });
+  launch(c, "cols_fwd_j", [&] { launch_cols(n, n, 0, -1, c->spec, c->spec, 1, 0, c->tw, s); });
+  // sweep 1: C = sum_R B_R FFT(phi_R)
+  launch(c, "diff_fill", [&] { launch_fill_c(c->df_c, n3, s); });
+  for (int rb = 0; rb < nr; rb += c->K) {
+    const int kk = std::min(c->K, nr - rb);
+    const int kp = (kk + 1) / 2;
+    launch(c, "filter_i", [&] {
+      launch_filter_i(n, c->spec, c->ubuf, kk, n3, c->wlut + (long)rb * c->mlut, c->mlut,
+                      c->d_kappa + rb, c->tw, s, c->tile_filter, 0);
+    });
+    launch(c, "cols_inv_j", [&] { launch_cols(n, n, 0, +1, c->ubuf, c->ubuf, kk, n3, c->tw, s); });
+    launch(c, "rows_phi", [&] { launch_rows_phi(n, c->ubuf, kk, n3, c->bal, c->df_st, inv_tau, c->tw, s); });
+    launch(c, "cols_fwd_j", [&] { launch_cols(n, n, 0, -1, c->ubuf, c->ubuf, kp, n3, c->tw, s); });
+    launch(c, "ball_acc_i", [&] {
+      launch_ball_i(n, true, c->ubuf, n3, c->df_c, 0, kk, c->df_ball + (long)rb * h3, c->tw, s);
+    });
+  }
+  launch(c, "cols_inv_j", [&] { launch_cols(n, n, 0, +1, c->df_c, c->df_c, 1, 0, c->tw, s); });
+  launch(c, "c2c_rows", [&] { launch_c2c_rows(n, +1, c->df_c, c->tw, s); });
+  // relaxed state, its P(k) and the observables' adjoints
+  launch(c, "relaxed_state", [&] {
+    launch_relaxed_state(n3, c->df_c, c->df_rec, c->df_ion, c->df_delta, c->df_x, c->df_t,
+                         c->df_fq, c->df_fa, c->df_part, c->df_st, s);
+  });
+  enqueue_pk(c, c->df_t, c->df_half, nbins);
+  const double dl = c->dx;
+  const double norm = dl * dl * dl / ((double)n3);
+  launch(c, "pk_adjoint", [&] {
+    launch_pk_coef(c->df_gp, c->pk_out, nbins, norm, c->df_coef, s);
+    launch_pk_adjoint(n, c->df_half, c->pk_bin, c->df_coef, s);
+  });
+  c2r3d(c, c->df_half, c->df_t);  // df_t = dL/dT
+  launch(c, "seed_a", [&] {
+    launch_seed_a(n3, c->df_t, c->df_delta, c->df_x, c->df_fq, c->df_fa, (float)(gx / (double)n3),
+                  c->df_c, c->df_qbar, c->df_dbar, s);
+  });
+  launch(c, "c2c_rows", [&] { launch_c2c_rows(n, -1, c->df_c, c->tw, s); });
+  launch(c, "cols_fwd_j", [&] { launch_cols(n, n, 0, -1, c->df_c, c->df_c, 1, 0, c->tw, s); });
+  // sweep 2: G = sum_R W_R FFT(seed_R)
+  launch(c, "diff_fill", [&] {
+    launch_fill_c(c->df_g, n3, s);
+    launch_fill_d(c->df_rho, rows_blocks(n), s);
+  });
+  for (int rb = 0; rb < nr; rb += c->K) {
+    const int kk = std::min(c->K, nr - rb);
+    const int kp = (kk + 1) / 2;
+    launch(c, "ball_i", [&] {
+      launch_ball_i(n, false, c->df_c, 0, c->df_v, n3, kk, c->df_ball + (long)rb * h3, c->tw, s);
+    });
+    launch(c, "cols_inv_j", [&] { launch_cols(n, n, 0, +1, c->df_v, c->df_v, kp, n3, c->tw, s); });
+    launch(c, "filter_i", [&] {
+      launch_filter_i(n, c->spec, c->ubuf, kk, n3, c->wlut + (long)rb * c->mlut, c->mlut,
+                      c->d_kappa + rb, c->tw, s, c->tile_filter, 0);
+    });
+    launch(c, "cols_inv_j", [&] { launch_cols(n, n, 0, +1, c->ubuf, c->ubuf, kk, n3, c->tw, s); });
+    launch(c, "rows_seed", [&] {
+      launch_rows_seed(n, c->ubuf, c->df_v, kk, n3, n3, c->bal, c->df_st, inv_tau, c->df_rho, c->tw, s);
+    });
+    launch(c, "cols_fwd_j", [&] { launch_cols(n, n, 0, -1, c->ubuf, c->ubuf, kk, n3, c->tw, s); });
+    launch(c, "filter_acc_i", [&] {
+      launch_filter_acc_i(n, c->ubuf, n3, kk, c->wlut + (long)rb * c->mlut, c->mlut, c->df_g, c->tw, s);
+    });
+  }
+  launch(c, "cols_inv_j", [&] { launch_cols(n, n, 0, +1, c->df_g, c->df_g, 1, 0, c->tw, s); });
+  launch(c, "c2c_rows", [&] { launch_c2c_rows(n, +1, c->df_g, c->tw, s); });
+  launch(c, "grad_cells", [&] {
+    launch_sum_rho(c->df_rho, rows_blocks(n), c->df_st, s);
+    launch_grad_cells(n3, c->df_g, c->df_qbar, c->df_rec, c->df_ion, emis, c->df_delta, c->df_dbar,
+                      c->df_st, c->df_dd, c->df_part, s);
+    launch_grad_g(n3, c->df_delta, c->df_dd, c->df_st, grad_g, s);
+  });
+}
+
+int diff_check(sf21_hii_ctx* c, const float* g, const float* emis, int nbins) {
+  if (!c || !g || !emis || nbins < 1 || nbins > 64) return SF21_ERR_ARG;
+  if (c->cfg.input_mode != SF21_INPUT_DIRECT || c->cfg.path != SF21_PATH_FUSED) return SF21_ERR_ARG;
+  if (c->n < 64 || !diff_size_supported(c->n)) return SF21_ERR_ARG;
+  if (hipSetDevice(c->cfg.device) != hipSuccess) return SF21_ERR_HIP;
+  (void)hipGetLastError();
+  int rc;
+  if ((rc = ensure_pk(c, nbins))) return rc;
+  return ensure_diff(c);
 }
 
 }  // namespace
@@ -677,43 +902,9 @@ int sf21_hii_power_spectrum_dev(sf21_hii_ctx* c, const float* field, int nbins, 
   if (hipSetDevice(c->cfg.device) != hipSuccess) return SF21_ERR_HIP;
   (void)hipGetLastError();
   int rc;
-  const int n = c->n, hh = n / 2;
-  if (c->pk_nbins != nbins) {
-    // bin of every integer |k|^2 (in dk units): linear bins of |k| over
-    // [1, n/2], width (n/2 - 1)/nbins, the last closed; k = 0 and the
-    // corners beyond n/2 are dropped (-1)
-    const long mmax = 3L * hh * hh;
-    std::vector<short> bins(mmax + 1);
-    std::vector<float> kab(mmax + 1);
-    const double width = (hh - 1) / (double)nbins;
-    for (long m = 0; m <= mmax; ++m) {
-      const double kq = std::sqrt((double)m);
-      int b = -1;
-      if (m >= 1 && kq <= hh) {
-        b = (int)std::floor((kq - 1.0) / width);
-        if (b >= nbins) b = nbins - 1;
-      }
-      bins[m] = (short)b;
-      kab[m] = (float)(kq * c->dk);
-    }
-    dev_free(c->pk_bin); dev_free(c->pk_kabs); dev_free(c->pk_part); dev_free(c->pk_out);
-    if ((rc = dev_alloc(&c->pk_bin, bins.size())) || (rc = dev_alloc(&c->pk_kabs, kab.size())) ||
-        (rc = dev_alloc(&c->pk_part, (size_t)3 * nbins * pk_partials_count(n))) ||
-        (rc = dev_alloc(&c->pk_out, (size_t)3 * nbins)))
-      return rc;
-    if (hipMemcpy(c->pk_bin, bins.data(), sizeof(short) * bins.size(), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(c->pk_kabs, kab.data(), sizeof(float) * kab.size(), hipMemcpyHostToDevice) != hipSuccess)
-      return SF21_ERR_HIP;
-    c->pk_nbins = nbins;
-  }
+  if ((rc = ensure_pk(c, nbins))) return rc;
   // the half spectrum goes to the context's spectrum buffer (n^3 >= n*n*(n/2+1))
-  r2c3d(c, field, c->spec);
-  const double dl = c->dx;
-  const double norm = dl * dl * dl / ((double)c->n3);
-  launch(c, "power_spectrum", [&] {
-    launch_power_spectrum(n, c->spec, c->pk_bin, c->pk_kabs, nbins, norm, c->pk_part, c->pk_out,
-                          c->stream);
-  });
+  enqueue_pk(c, field, c->spec, nbins);
   std::vector<double> host(3 * nbins);
   if (hipMemcpyAsync(host.data(), c->pk_out, sizeof(double) * 3 * nbins, hipMemcpyDeviceToHost,
                      c->stream) != hipSuccess ||
@@ -852,4 +1043,55 @@ int sf21_fft_destroy(sf21_fft_plan* p) {
   return SF21_OK;
 }
 
+
+// ---- differentiable mode ---------------------------------------------------
+int sf21_hii_diff_forward_dev(sf21_hii_ctx* c, const float* g, const float* emis, double zeta,
+                              int nbins, float* x_out, double* mean_x, double* pk) {
+  int rc = diff_check(c, g, emis, nbins);
+  if (rc) return rc;
+  float* x = x_out ? x_out : c->df_x;
+  enqueue_diff_forward(c, g, emis, zeta, nbins, x);
+  if (hipMemcpyAsync(c->df_host, c->df_st, sizeof(double) * DS_COUNT, hipMemcpyDeviceToHost,
+                     c->stream) != hipSuccess ||
+      hipMemcpyAsync(c->df_host + DS_COUNT, c->pk_out, sizeof(double) * nbins,
+                     hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess)
+    return SF21_ERR_HIP;
+  if (mean_x) *mean_x = c->df_host[DS_SUM_X] / (double)c->n3;
+  if (pk)
+    for (int b = 0; b < nbins; ++b) pk[b] = c->df_host[DS_COUNT + b];
+  if (c->prof) collect(c);
+  return check_hip();
+}
+
+int sf21_hii_diff_backward_dev(sf21_hii_ctx* c, const float* g, const float* emis, double zeta,
+                               double tau, int nbins, double dl_dx, const double* dl_dpk,
+                               float* grad_g, double* grad_zeta, double* relaxed_mean_x,
+                               double* relaxed_pk) {
+  int rc = diff_check(c, g, emis, nbins);
+  if (rc) return rc;
+  if (!grad_g || !(tau > 0.0)) return SF21_ERR_ARG;
+  for (int b = 0; b < nbins; ++b) c->df_host[b] = dl_dpk ? dl_dpk[b] : 0.0;
+  if (hipMemcpyAsync(c->df_gp, c->df_host, sizeof(double) * nbins, hipMemcpyHostToDevice,
+                     c->stream) != hipSuccess)
+    return SF21_ERR_HIP;
+  enqueue_diff_backward(c, g, emis, zeta, tau, nbins, dl_dx, grad_g);
+  // the staging buffer is reused for the results only after the stream has
+  // consumed the upload above (same stream, ordered)
+  if (hipMemcpyAsync(c->df_host + 64, c->df_st, sizeof(double) * DS_COUNT, hipMemcpyDeviceToHost,
+                     c->stream) != hipSuccess ||
+      hipMemcpyAsync(c->df_host + 64 + DS_COUNT, c->pk_out, sizeof(double) * nbins,
+                     hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess)
+    return SF21_ERR_HIP;
+  const double* hs = c->df_host + 64;
+  if (grad_zeta) *grad_zeta = hs[DS_ZETA_BAR];
+  if (relaxed_mean_x) *relaxed_mean_x = hs[DS_SUM_XT] / (double)c->n3;
+  if (relaxed_pk)
+    for (int b = 0; b < nbins; ++b) relaxed_pk[b] = hs[DS_COUNT + b];
+  if (c->prof) collect(c);
+  return check_hip();
+}
+
 }  // extern "C"
+

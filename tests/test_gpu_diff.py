@@ -1,0 +1,124 @@
+"""GPU checks of the differentiable mode (sf21_hii_diff_forward_dev /
+sf21_hii_diff_backward_dev) at 64^3, side 100 Mpc, 29 radii (60 Mpc * n/128
+down by 0.9 to the cell size), zeta = 50 (<x> ~ 0.5):
+
+  * the relaxed adjoint against autograd of the FP64 torch reference
+    (tests/diff_reference.py) at tau in {0.05, 0.02, 0.01}: gradient in g
+    (relative L2 error), gradient in zeta, relaxed <x~> and P_b;
+  * central differences of the GPU's own relaxed loss along zeta (step 1e-3);
+  * the exact forward: <x> and P_b against the FP64 oracle's bubble finder
+    on the same fields (FP32-undecidable flips allowed through the mean);
+  * tau -> 0: the relaxed <x~> tends to the exact <x>.
+"""
+import numpy as np
+import pytest
+import torch
+
+import diff_reference as dr
+import oracle_bind as orc
+from simfast21_amd import INPUT_DIRECT, PATH_FUSED, HiiContext, synth
+
+pytestmark = pytest.mark.gpu
+
+N = 64
+ZETA = 50.0
+NB = 20
+
+
+def _case(n=N, seed=5):
+    side = 200.0 * n / 128
+    g = synth.gaussian_field(n, -2.5, 1.2, seed).astype(np.float64)
+    delta = np.exp(g) / np.exp(g).mean() - 1
+    n_src = int(round(4000 * (n / 128) ** 3))
+    E = synth.point_sources(delta.astype(np.float32), n_src, seed)
+    radii = synth.geometric_ladder(60.0 * n / 128, 0.9, side / n)
+    return side, radii, g, E
+
+
+@pytest.fixture(scope="module")
+def setup(torch_cuda):
+    side, radii, g, E = _case()
+    ctx = HiiContext(N, side, radii, INPUT_DIRECT, PATH_FUSED)
+    g32 = g.astype(np.float32)
+    gd = torch.from_numpy(g32.ravel()).cuda()
+    ed = torch.from_numpy(E.ravel()).cuda()
+    # the reference sees exactly the float32 inputs the GPU sees
+    ref = dr.Setup(N, side, radii, E.astype(np.float64))
+    yield dict(ctx=ctx, side=side, radii=radii, g=g32.astype(np.float64), E=E, gd=gd, ed=ed, ref=ref)
+    ctx.close()
+
+
+def _weights(ref, g, seed=11):
+    """dL/dP_b scaled by the bin values so every bin weighs O(1)."""
+    with torch.no_grad():
+        _, P, _ = dr.relaxed(ref, torch.from_numpy(g.reshape(N, N, N)), ZETA, 0.05)
+    rng = np.random.default_rng(seed)
+    return rng.standard_normal(NB) / P.numpy()
+
+
+@pytest.mark.parametrize("tau", [0.05, 0.02, 0.01])
+def test_adjoint_matches_fp64_reference(setup, tau):
+    s = setup
+    cp = _weights(s["ref"], s["g"])
+    cx = 2.0
+    grad, gz, mx, pk = s["ctx"].diff_backward(s["gd"], s["ed"], ZETA, tau, cx, cp, NB)
+    grad = grad.cpu().numpy().astype(np.float64)
+    L, gg, gzr = dr.loss_and_grad(s["ref"], s["g"], ZETA, tau, cx, cp, NB)
+    with torch.no_grad():
+        X, P, _ = dr.relaxed(s["ref"], torch.from_numpy(s["g"].reshape(N, N, N)), ZETA, tau, NB)
+    X, P = float(X), P.numpy()
+    rel_g = np.linalg.norm(grad - gg) / np.linalg.norm(gg)
+    cos = float(grad @ gg / (np.linalg.norm(grad) * np.linalg.norm(gg)))
+    print(f"\ntau={tau}: |dg|/|g| {rel_g:.2e} cos {cos:.8f}  zeta' gpu {gz:.6e} ref {gzr:.6e}  "
+          f"<x~> gpu {mx:.8f} ref {X:.8f}  P max rel {np.max(np.abs(pk / P - 1)):.2e}", flush=True)
+    assert abs(mx - X) < 2e-5
+    assert np.max(np.abs(pk / P - 1)) < 2e-4
+    assert abs(gz - gzr) <= 2e-3 * abs(gzr)
+    assert rel_g < 5e-3
+    assert cos > 0.99999
+    # Delta = exp(g)/<exp(g)> is invariant under a uniform shift of g
+    assert abs(grad.sum()) <= 1e-3 * np.abs(grad).sum()
+
+
+def test_zeta_gradient_vs_central_differences(setup):
+    """Central differences of the GPU's relaxed loss, step 1e-3 in zeta."""
+    s = setup
+    tau = 0.05
+    cp = _weights(s["ref"], s["g"])
+    ctx = s["ctx"]
+
+    def loss(z):
+        _, _, mx, pk = ctx.diff_backward(s["gd"], s["ed"], z, tau, 1.0, cp, NB)
+        return mx + float(np.dot(cp, pk))
+
+    _, gz, _, _ = ctx.diff_backward(s["gd"], s["ed"], ZETA, tau, 1.0, cp, NB)
+    h = 1e-3
+    fd = (loss(ZETA + h) - loss(ZETA - h)) / (2 * h)
+    print(f"\nzeta: adjoint {gz:.6e} central difference {fd:.6e}", flush=True)
+    assert abs(fd - gz) <= 2e-2 * abs(gz)
+
+
+def test_exact_forward_matches_oracle(setup):
+    s = setup
+    x, mx, pk = s["ctx"].diff_forward(s["gd"], s["ed"], ZETA, NB)
+    x = x.cpu().numpy().astype(np.float64)
+    d1 = np.exp(s["g"]) / np.exp(s["g"]).mean()
+    rec = (d1 ** 2).astype(np.float32)
+    ion = (ZETA * s["E"].astype(np.float64)).astype(np.float32)
+    xo, neu, _ = orc.hii_slice(N, s["side"], s["radii"], rec, ion.ravel(), None, 64, 1)
+    ndiff = int(np.sum(np.abs(x - xo) > 1e-4))
+    print(f"\nexact <x>: gpu {mx:.8f} oracle {1 - neu:.8f}; cells differing {ndiff}", flush=True)
+    assert abs(mx - float(np.mean(x))) < 1e-7
+    assert abs(mx - (1 - neu)) < 1e-3
+    # P(k) of T = (1 - x) Delta from the GPU's own x, FP64 on the host
+    T = torch.from_numpy(((1 - x) * d1.ravel()).reshape(N, N, N))
+    P = dr.power_spectrum(T, s["side"], NB).numpy()
+    assert np.max(np.abs(pk / P - 1)) < 1e-4
+
+
+def test_relaxed_tends_to_exact(setup):
+    s = setup
+    _, mx_exact, _ = s["ctx"].diff_forward(s["gd"], s["ed"], ZETA, NB)
+    _, _, mx, _ = s["ctx"].diff_backward(s["gd"], s["ed"], ZETA, 1e-3, 1.0, None, NB)
+    print(f"\n<x~>(tau=1e-3) {mx:.6f}  exact <x> {mx_exact:.6f}", flush=True)
+    assert abs(mx - mx_exact) < 2e-3
