@@ -11,6 +11,8 @@ with P ~ (1+delta_s)^2 and lognormal strengths, rec_s = (1+delta_s)^2.
 A step is one pass of all radii over all slices (inputs resident in HBM).
 `value` = slices * N^3 * n_radii / step time.  The plain per-radius GPU path
 (the reference's dataflow) is timed on one slice and scaled to ten slices.
+The differentiable mode (one redshift at 512^3, same radii) reports the exact
+forward's time and the relaxed backward's, and their ratio, per tau.
 """
 from __future__ import annotations
 
@@ -30,11 +32,9 @@ METRIC = "voxel·radius updates/s at 512^3 (batched vs plain GPU); grad/forward 
 HBM_PEAK_GBS = 8000.0
 
 
-def make_inputs(torch, n, side, n_slices, n_src, seed=3001, slope=-2.5, sigma=1.2):
-    """Synthetic slices on the device (float32 boxes, reference layout)."""
+def gaussian_field_dev(torch, gen, n, slope=-2.5, sigma=1.2):
+    """Gaussian field with P(k) ~ k^slope, std sigma (float32, device)."""
     dev = "cuda"
-    gen = torch.Generator(device=dev)
-    gen.manual_seed(seed)
     white = torch.randn((n, n, n), generator=gen, device=dev, dtype=torch.float32)
     spec = torch.fft.rfftn(white)
     del white
@@ -46,27 +46,107 @@ def make_inputs(torch, n, side, n_slices, n_src, seed=3001, slope=-2.5, sigma=1.
     del amp, k2
     g = torch.fft.irfftn(spec, s=(n, n, n))
     del spec
-    g = (g - g.mean()) / g.std() * sigma
+    return (g - g.mean()) / g.std() * sigma
+
+
+def sources_dev(torch, gen, delta1, n_src):
+    """Unit-efficiency emissivity: n_src sources with P(cell) ~ (1+delta)^2,
+    lognormal(0, 1) strengths (float32, device, flat)."""
+    dev = "cuda"
+    n3 = delta1.numel()
+    w = (delta1 ** 2).reshape(-1)
+    cdf = torch.cumsum(w, 0)
+    cdf /= cdf[-1].clone()
+    u = torch.rand(n_src, generator=gen, device=dev, dtype=torch.float64)
+    cells = torch.searchsorted(cdf, u).clamp_max(n3 - 1)
+    strength = torch.exp(torch.randn(n_src, generator=gen, device=dev, dtype=torch.float64))
+    emis = torch.zeros(n3, device=dev, dtype=torch.float64)
+    emis.index_add_(0, cells, strength)
+    return emis.float().contiguous()
+
+
+def make_inputs(torch, n, side, n_slices, n_src, seed=3001, slope=-2.5, sigma=1.2):
+    """Synthetic slices on the device (float32 boxes, reference layout)."""
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed)
+    g = gaussian_field_dev(torch, gen, n, slope, sigma)
     growth = np.linspace(0.10, 0.25, n_slices)
-    recs, ions, means = [], [], []
+    recs, ions = [], []
     for s in range(n_slices):
         gs = g * float(growth[s] / 0.25)
         e = torch.exp(gs.double())
         e /= e.mean()
-        delta1 = e  # 1 + delta
-        w = (delta1 ** 2).reshape(-1)
-        cdf = torch.cumsum(w, 0)
-        cdf /= cdf[-1].clone()
-        u = torch.rand(n_src, generator=gen, device=dev, dtype=torch.float64)
-        cells = torch.searchsorted(cdf, u).clamp_max(n ** 3 - 1)
-        strength = torch.exp(torch.randn(n_src, generator=gen, device=dev, dtype=torch.float64))
-        emis = torch.zeros(n ** 3, device=dev, dtype=torch.float64)
-        emis.index_add_(0, cells, strength)
-        recs.append((delta1 ** 2).float().reshape(-1).contiguous())
-        ions.append(emis.float().contiguous())
-        del e, w, cdf, emis, delta1
+        ions.append(sources_dev(torch, gen, e, n_src))
+        recs.append((e ** 2).float().reshape(-1).contiguous())
+        del e
     torch.cuda.synchronize()
     return recs, ions
+
+
+def bench_diff(torch, sf, n, side, radii, n_src, taus=(0.05, 0.02, 0.01), reps=3, seed=5005):
+    """Differentiable mode at one redshift: Gaussian g (sigma 1.2, P ~ k^-2.5),
+    emissivity pattern from n_src sources ~ (1+delta)^2, zeta set for
+    <x> ~ 0.5.  Times the exact forward (fields, bubble finder, T, P(k)) and
+    the relaxed backward (recomputing every per-radius field) per tau."""
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed)
+    g = gaussian_field_dev(torch, gen, n).reshape(-1).contiguous()
+    e = torch.exp(g.double())
+    e /= e.mean()
+    emis = sources_dev(torch, gen, e, n_src)
+    del e
+    ctx = sf.HiiContext(n, side, radii, sf.INPUT_DIRECT, sf.PATH_FUSED)
+    x = torch.empty_like(g)
+    grad = torch.empty_like(g)
+    # zeta for <x> ~ 0.5 (bisection in log zeta on the exact forward)
+    lo, hi = 1e-2, 1e6
+    zeta, mx = 1.0, 0.0
+    for _ in range(14):
+        zeta = math.sqrt(lo * hi)
+        _, mx, _ = ctx.diff_forward(g, emis, zeta, 20, x_out=x)
+        if abs(mx - 0.5) < 0.02:
+            break
+        if mx < 0.5:
+            lo = zeta
+        else:
+            hi = zeta
+    rng = np.random.default_rng(7)
+    _, _, pk = ctx.diff_forward(g, emis, zeta, 20, x_out=x)
+    cp = rng.standard_normal(20) / np.where(pk > 0, pk, 1.0)
+    ctx.diff_backward(g, emis, zeta, taus[0], 1.0, cp, 20, grad_out=grad)  # warm: ball tables
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.diff_forward(g, emis, zeta, 20, x_out=x)
+    torch.cuda.synchronize()
+    fwd_ms = (time.perf_counter() - t0) * 1e3 / reps
+    bwd = {}
+    for tau in taus:
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            _, gz, mxt, _ = ctx.diff_backward(g, emis, zeta, tau, 1.0, cp, 20, grad_out=grad)
+        torch.cuda.synchronize()
+        bwd[str(tau)] = round((time.perf_counter() - t0) * 1e3 / reps, 2)
+        print(f"[bench] diff tau={tau}: backward {bwd[str(tau)]:.1f} ms, <x~> {mxt:.4f}, "
+              f"dL/dzeta {gz:.4e}", file=sys.stderr, flush=True)
+    # per-kernel split of one backward
+    ctx.reset_stats()
+    ctx.profile(True)
+    ctx.diff_backward(g, emis, zeta, taus[0], 1.0, cp, 20, grad_out=grad)
+    kst = ctx.kernel_stats()
+    ctx.profile(False)
+    ctx.close()
+    worst = max(bwd.values())
+    return {
+        "workload": (f"differentiable mode, {n}^3 float32, side {side:g} Mpc, one redshift, "
+                     f"{len(radii)} radii, {n_src} sources, zeta {zeta:.4g} (<x> {mx:.3f}), "
+                     f"20 P(k) bins"),
+        "forward_ms": round(fwd_ms, 2),
+        "backward_ms": bwd,
+        "grad_over_forward": {k: round(v / fwd_ms, 3) for k, v in bwd.items()},
+        "grad_over_forward_max": round(worst / fwd_ms, 3),
+        "backward_kernels_ms": {k: [c, round(t, 3)] for k, (c, t) in kst.items()},
+    }
 
 
 def cpu_baseline(n, side, radii_full, seconds_hint=20.0):
@@ -109,6 +189,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--radius-batch", type=int, default=0)
+    ap.add_argument("--no-diff", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: the bubble finder's slices are independent and a "
@@ -204,6 +285,15 @@ def main():
         print(f"[bench] plain {pms:.1f} ms/slice", file=sys.stderr, flush=True)
         del pctx
 
+    diff = None
+    if not args.no_diff:
+        try:
+            diff = bench_diff(torch, sf, n, side, radii, n_src)
+        finally:
+            torch.cuda.empty_cache()
+        print(f"[bench] diff forward {diff['forward_ms']:.1f} ms, grad/forward "
+              f"{diff['grad_over_forward']}", file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu:
         try:
@@ -235,6 +325,7 @@ def main():
         "profiled_ms_per_step": round(prof_ms, 3),
         "kernels_ms": {k: [c, round(t, 3)] for k, (c, t) in stats.items()},
         "plain": plain,
+        "diff": diff,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -129,7 +129,8 @@ int sf21_hii_power_spectrum_dev(sf21_hii_ctx *ctx, const float *field, int nbins
  * pk (host, nbins, nullable).
  * diff_backward returns the gradient of L = dl_dx <x~> + sum_b dl_dpk[b] P_b
  * of the tau-relaxed model (the centre test min(I_R - R_R, I_R) >= 0 becomes
- * softplus(s_R/tau) with s_R = min(I_R - R_R, I_R)/max(R_R, 1e-3 <rec>);
+ * phi_R = softplus(u) - softplus(u - 20), u = s_R/tau, with
+ * s_R = min(I_R - R_R, I_R)/max(R_R, 1e-3 <rec>);
  * the union of balls becomes x~ = q + (1 - q)(1 - exp(-sum_R B_R * phi_R)),
  * B_R the exact ball, q the quick fill) with respect to g (grad_g, device
  * n^3) and zeta (*grad_zeta), plus the relaxed <x~> and P_b (host,

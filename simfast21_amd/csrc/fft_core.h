@@ -118,6 +118,16 @@ struct Dft<5, SIGN> {
 // clobber pins every LDS access of an exchange on its side of the barrier.
 __device__ __forceinline__ void lds_pin() { asm volatile("" ::: "memory"); }
 
+// Returns v unchanged but opaque to the optimiser.  Used on the lane index
+// at the top of radius loops: without it the compiler hoists every FFT
+// stage's LDS and twiddle addresses out of the loop and keeps them live in
+// VGPRs (124+ VGPRs, spills at 1024-thread blocks); recomputing them costs a
+// few VALU per iteration and frees half the register file.
+__device__ __forceinline__ int opaque_lane(int v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
 // LDS index padding: one slot every 8 breaks the stride-R write pattern of
 // the first stage.
 __device__ __forceinline__ int lds_pad(int i) { return i + (i >> 3); }

@@ -4,8 +4,13 @@
 //   s_R   = min(I_R - R_R, I_R) / max(R_R, rho),  rho = 1e-3 <rec>
 //           (R_R, I_R: rec and ion filtered by the top hat W_R; the sign of
 //            s_R is the centre test of get_HIIbubbles.c:339-349)
-//   A     = sum_R B_R (*) softplus(s_R / tau)   (B_R: the lattice ball of
-//           radius R, (*) periodic convolution)
+//   phi_R = softplus(u) - softplus(u - c), u = s_R / tau, c = kPhiCap:
+//           a softplus bounded by c (one covering centre gives 1 - exp(-c));
+//           unbounded, the centres' phi (up to ~1e6) would set the FP32
+//           error of the ball convolution far above the small A values of
+//           the transition region
+//   A     = sum_R B_R (*) phi_R   (B_R: the lattice ball of radius R, (*)
+//           periodic convolution)
 //   x~    = q + (1 - q)(1 - exp(-A))            (q: quick fill, :281-288)
 //   T     = (1 - x~) Delta,   observables <x~> and the binned P(k) of T.
 //
@@ -156,15 +161,6 @@ __global__ void k_ball_fold(int n, const float2* __restrict__ spec, float* __res
 
 __device__ __forceinline__ int fold_idx(int i, int n) { return i > (n >> 1) ? n - i : i; }
 
-// Returns v unchanged but opaque to the optimiser.  Used on the lane index
-// inside radius loops: without it the compiler hoists every stage's LDS and
-// twiddle addresses out of the loop and keeps them live in VGPRs (spilling
-// at 1024-thread blocks); recomputing them costs a few VALU per iteration.
-__device__ __forceinline__ int opaque(int v) {
-  asm volatile("" : "+v"(v));
-  return v;
-}
-
 // ---------------------------------------------------------------------------
 // Relaxation helpers
 // ---------------------------------------------------------------------------
@@ -172,6 +168,10 @@ __device__ __forceinline__ float softplus_f(float v) {
   return v > 20.0f ? v : (v < -30.0f ? expf(v) : log1pf(expf(v)));
 }
 __device__ __forceinline__ float sigmoid_f(float v) { return 1.0f / (1.0f + expf(-v)); }
+
+constexpr float kPhiCap = 20.0f;
+__device__ __forceinline__ float phi_of(float u) { return softplus_f(u) - softplus_f(u - kPhiCap); }
+__device__ __forceinline__ float dphi_of(float u) { return sigmoid_f(u) - sigmoid_f(u - kPhiCap); }
 
 // rho in unnormalised units: N rho = 1e-3 sum(rec)
 __device__ __forceinline__ float rho_u_of(const double* st) {
@@ -201,7 +201,7 @@ __global__ __launch_bounds__(256) void k_rows_phi(float2* u, int nrad, long ustr
   for (int m = 0; m < E; ++m) pa[m] = 0.0f;
 #pragma unroll 1
   for (int r = 0; r < nrad; ++r) {
-    const int tl = opaque(t);
+    const int tl = opaque_lane(t);
     float2 v[E];
     const float2* src = u + (long)r * ustride + row * L;
 #pragma unroll
@@ -211,7 +211,7 @@ __global__ __launch_bounds__(256) void k_rows_phi(float2* u, int nrad, long ustr
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       const float ru = v[m].x, iu = v[m].y * inv_s;
-      ph[m] = softplus_f(fminf(iu - ru, iu) / fmaxf(ru, rho_u) * inv_tau);
+      ph[m] = phi_of(fminf(iu - ru, iu) / fmaxf(ru, rho_u) * inv_tau);
     }
     if ((r & 1) == 0 && r + 1 < nrad) {
 #pragma unroll
@@ -261,7 +261,7 @@ __global__ __launch_bounds__(TILE*(L / Plan<L>::E)) void k_ball_i(
   }
 #pragma unroll 1
   for (int pr = 0; pr < npair; ++pr) {
-    const int tl = opaque(t);
+    const int tl = opaque_lane(t);
     const float* __restrict__ ta = bt + (long)(2 * pr) * h3;
     const float* __restrict__ tb = bt + (long)(2 * pr + 1) * h3;
     const bool two = 2 * pr + 1 < nrad;
@@ -305,7 +305,7 @@ __global__ __launch_bounds__(TILE*(L / Plan<L>::E)) void k_ball_i(
 // ---------------------------------------------------------------------------
 // Sweep 2, rows: inverse along p of the recomputed pair (N R_R, s N I_R) and
 // of the ball pair (N phibar_a, N phibar_b); seeds of the normalised fields
-//   sbar = phibar sigmoid(s/tau)/tau,
+//   sbar = phibar phi'(s/tau)/tau,  phi'(u) = sigmoid(u) - sigmoid(u - c),
 //   Rbar = sbar ds/dR,  Ibar = sbar ds/dI,
 //   ds/dI = 1/den, ds/dR = -[R >= 0]/den - [R > rho] num/den^2,
 //   num = min(I - R, I), den = max(R, rho)
@@ -336,7 +336,7 @@ __global__ __launch_bounds__(256) void k_rows_seed(float2* u, const float2* v, i
   for (int pr = 0; pr < npair; ++pr) {
     float2 pb[E];
     {
-      const int tl = opaque(t);
+      const int tl = opaque_lane(t);
       const float2* bs = v + (long)pr * vstride + row * L;
 #pragma unroll
       for (int m = 0; m < E; ++m) pb[m] = bs[tl + TPL * m];
@@ -345,7 +345,7 @@ __global__ __launch_bounds__(256) void k_rows_seed(float2* u, const float2* v, i
     const int nh = (2 * pr + 1 < nrad) ? 2 : 1;
 #pragma unroll 1
     for (int hh = 0; hh < nh; ++hh) {
-      const int tl = opaque(t);
+      const int tl = opaque_lane(t);
       float2 y[E];
       float2* src = u + (long)(2 * pr + hh) * ustride + row * L;
 #pragma unroll
@@ -362,7 +362,7 @@ __global__ __launch_bounds__(256) void k_rows_seed(float2* u, const float2* v, i
         const float inv_den = 1.0f / den;
         const float s = num * inv_den;
         const float phib = hh ? pb[m].y : pb[m].x;
-        const float sb = phib * sigmoid_f(s * inv_tau) * inv_tau;
+        const float sb = phib * dphi_of(s * inv_tau) * inv_tau;
         const float q = num * inv_den * inv_den;
         const float dR = (use_a ? -inv_den : 0.0f) - (den_r ? q : 0.0f);
         y[m] = make_float2(sb * dR, sb * inv_den);
@@ -399,7 +399,7 @@ __global__ __launch_bounds__(TILE*(L / Plan<L>::E)) void k_filter_acc_i(
   for (int m = 0; m < E; ++m) z[m] = make_float2(0.f, 0.f);
 #pragma unroll 1
   for (int r = 0; r < nrad; ++r) {
-    const int tl = opaque(t);
+    const int tl = opaque_lane(t);
     const float2* src = in + (long)r * istride + off;
     float2 v[E];
 #pragma unroll

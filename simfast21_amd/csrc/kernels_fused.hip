@@ -251,26 +251,27 @@ __global__ __launch_bounds__(TILE*(L / Plan<L>::E)) void k_filter_i(
   const int mjp = fold_sq(kj, L) + fold_sq(kp, L);
 #pragma unroll 1
   for (int r = 0; r < nrad; ++r) {
+    const int tl = opaque_lane(t);
     float2 v[E];
     if constexpr (WTAB) {
       const float* __restrict__ w = wlut + (long)r * mlut;
 #pragma unroll
       for (int m = 0; m < E; ++m) {
-        const float wv = w[mjp + fold_sq(t + TPL * m, L)];
+        const float wv = w[mjp + fold_sq(tl + TPL * m, L)];
         v[m] = make_float2(z[m].x * wv, z[m].y * wv);
       }
     } else {
       const float kr = kappa[r];  // dk * R
 #pragma unroll
       for (int m = 0; m < E; ++m) {
-        const float wv = w_tophat(kr * sqrtf((float)(mjp + fold_sq(t + TPL * m, L))));
+        const float wv = w_tophat(kr * sqrtf((float)(mjp + fold_sq(tl + TPL * m, L))));
         v[m] = make_float2(z[m].x * wv, z[m].y * wv);
       }
     }
-    line_fft<L, +1>(v, sre, sim, t, tw);
+    line_fft<L, +1>(v, sre, sim, tl, tw);
     float2* __restrict__ dst = u + (long)r * ustride + (long)kj * L + kp;
 #pragma unroll
-    for (int m = 0; m < E; ++m) dst[(t + TPL * m) * (L * L)] = v[m];
+    for (int m = 0; m < E; ++m) dst[(tl + TPL * m) * (L * L)] = v[m];
   }
 }
 
@@ -303,17 +304,18 @@ __global__ __launch_bounds__(256) void k_inv_rows_crit(const float2* __restrict_
   for (int m = 0; m < E; ++m) wbest[m] = -1;
 #pragma unroll 1
   for (int r = 0; r < nrad; ++r) {
+    const int tl = opaque_lane(t);
     const float2* __restrict__ src = vin + (long)r * vstride + row * L;
     float2 v[E];
 #pragma unroll
-    for (int m = 0; m < E; ++m) v[m] = src[t + TPL * m];
-    line_fft<L, +1>(v, sre, sim, t, tw);
+    for (int m = 0; m < E; ++m) v[m] = src[tl + TPL * m];
+    line_fft<L, +1>(v, sre, sim, tl, tw);
     const int w = d2max[r];
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       // Re = N^3 rec_R, Im = s N^3 ion_R: centre test on (s rec_R, s ion_R)
       if (is_centre(v[m].x * sc, v[m].y)) wbest[m] = max(wbest[m], w);
-      if (dbg) dbg[(long)r * vstride + row * L + t + TPL * m] = make_float2(v[m].x, v[m].y / sc);
+      if (dbg) dbg[(long)r * vstride + row * L + tl + TPL * m] = make_float2(v[m].x, v[m].y / sc);
     }
   }
 #pragma unroll

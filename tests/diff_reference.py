@@ -15,7 +15,13 @@ Model (one slice, direct inputs):
 exact forward:   x = 1 where a ball B_R around a centre (s_R >= 0) covers the
                  cell, else q                       (:358-393, :428-458)
 relaxed backward (temperature tau):
-    A       = sum_R B_R (*) softplus(s_R / tau)      ((*) periodic convolution,
+    phi_R   = softplus(u) - softplus(u - c), u = s_R / tau, c = PHI_CAP = 20
+                                                     (a softplus bounded by c: one centre
+                                                      covering a cell already gives
+                                                      1 - exp(-c) = 1 - 2e-9; the bound keeps
+                                                      the FP32 ball convolution on the GPU
+                                                      accurate where A is small)
+    A       = sum_R B_R (*) phi_R                    ((*) periodic convolution,
                                                        B_R the exact ball |d|^2 <= d2max(R))
     D       = 1 - exp(-A)                            (soft union of the balls)
     x~      = q + (1 - q) D                          (-> the exact x as tau -> 0)
@@ -30,6 +36,7 @@ import numpy as np
 import torch
 
 PI = 3.1415926535897932385
+PHI_CAP = 20.0
 
 
 def d2max(R, dx):
@@ -107,7 +114,8 @@ def relaxed(setup, g, zeta, tau, nbins=20, exact_x=None):
         zr = torch.fft.ifftn(W * Z)
         rec_r, ion_r = zr.real, zr.imag
         s = torch.minimum(ion_r - rec_r, ion_r) / torch.maximum(rec_r, rho)
-        phi = torch.nn.functional.softplus(s / tau)
+        u = s / tau
+        phi = torch.nn.functional.softplus(u) - torch.nn.functional.softplus(u - PHI_CAP)
         A = A + torch.fft.ifftn(B * torch.fft.fftn(phi)).real
     D = 1.0 - torch.exp(-A)
     pos = (ion > 0) & (rec > 0)
