@@ -156,7 +156,7 @@ def cpu_baseline(n, side, radii_full, seconds_hint=20.0):
     import oracle_bind as orc
     from simfast21_amd import synth
     delta, emis, rec = synth.direct_boxes(n, side, 250000, 3001, growth=0.175 / 0.25)
-    nr = 2
+    nr = 12
     t0 = time.perf_counter()
     orc.hii_slice(n, side, radii_full[:nr], rec, emis, precision=32, dilation=0)
     dt = time.perf_counter() - t0
@@ -190,6 +190,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--radius-batch", type=int, default=0)
     ap.add_argument("--no-diff", action="store_true")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="also run one 1 GiB device copy (torch copy_) as the HBM-counter reference")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: the bubble finder's slices are independent and a "
@@ -211,6 +213,15 @@ def main():
     recs, ions = make_inputs(torch, n, side, S, n_src)
     outs = [torch.empty_like(r) for r in recs]
 
+    if args.calib_copy:
+        # HBM-counter references of known size: 1 GiB read + 1 GiB written by
+        # an elementwise kernel (torch mul) and by the runtime's copy
+        src = torch.ones(1 << 28, device="cuda")
+        dst = torch.empty_like(src)
+        torch.mul(src, 1.0, out=dst)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     ctx = sf.HiiContext(n, side, radii, sf.INPUT_DIRECT, sf.PATH_FUSED,
                         radius_batch=args.radius_batch)
     torch.cuda.synchronize()
@@ -240,7 +251,7 @@ def main():
     prof_ms = (time.perf_counter() - tp0) * 1e3 / prof_steps
     stats = ctx.kernel_stats()
     ctx.profile(False)
-    K = ctx_k = min(8 if args.radius_batch <= 0 else args.radius_batch, len(radii))
+    K = ctx.radius_batch
     # algorithmic HBM bytes per launch of each fused kernel (complex64 = 8 B/voxel)
     v = n ** 3
     nb = math.ceil(len(radii) / K)

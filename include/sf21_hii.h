@@ -63,11 +63,14 @@ typedef struct sf21_hii_config {
   int path;              /* SF21_PATH_* */
   double hubble;         /* h, SFR mode (Rrec's cell volume, user_functions.c:62) */
   double fesc;           /* escape fraction, SFR mode (get_HIIbubbles.c:273) */
-  int radius_batch;      /* radii per fused filter pass, 0 = automatic */
+  int radius_batch;      /* radii per fused filter pass, 0 = automatic (16, fewer if the
+                            K n^3 complex buffer does not fit) */
   int device;            /* HIP device ordinal */
   int filter_tile;       /* columns per block of the fused filter pass: 4, 8, 16 or 0 (auto) */
-  int filter_w;          /* top-hat weights: 0 = FP64 table per integer |k|^2 rounded to float,
-                            1 = FP32 evaluation in the kernel */
+  int filter_w;          /* top-hat weights W(|k| R): 0 = automatic (FP32 in the kernel),
+                            1 = FP32 evaluation in the kernel (series below |k|R = 1.25,
+                            3 (sin x - x cos x)/x^3 above), 2 = FP64 table per integer
+                            |k|^2 rounded to float */
   int n_halo;            /* N_halo, halo mode: the catalog grid (>= n) */
 } sf21_hii_config;
 
@@ -152,6 +155,10 @@ int sf21_hii_weight_map(sf21_hii_ctx *ctx, int *host_out);
  * get_HIIbubbles.c:324-325 gives them. */
 int sf21_hii_filtered_dev(sf21_hii_ctx *ctx, const float *a, const float *b, const float *c,
                           double z, int radius_index, float *out);
+
+/* Effective radii per fused filter pass (the automatic choice depends on
+ * free device memory); negative on error. */
+int sf21_hii_radius_batch(const sf21_hii_ctx *ctx);
 
 /* Kernel timing with HIP events on the context's stream. */
 int sf21_hii_profile(sf21_hii_ctx *ctx, int enable);

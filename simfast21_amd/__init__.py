@@ -112,6 +112,7 @@ def _lib():
     L.sf21_hii_weight_map.argtypes = [vp, vp]
     L.sf21_hii_filtered_dev.argtypes = [vp, vp, vp, vp, d, i, vp]
     L.sf21_hii_profile.argtypes = [vp, i]
+    L.sf21_hii_radius_batch.argtypes = [vp]
     L.sf21_hii_kernel_stats.argtypes = [vp, i, ctypes.c_char_p, i, ctypes.POINTER(ctypes.c_long), dp]
     L.sf21_hii_reset_stats.argtypes = [vp]
     L.sf21_hii_power_spectrum_dev.argtypes = [vp, vp, i, vp, vp, vp]
@@ -319,6 +320,10 @@ class HiiContext:
         _check(_lib().sf21_hii_filtered_dev(self._h, _ptr(a), _ptr(b), _ptr(c), float(z),
                                             int(radius_index), _ptr(out)), "sf21_hii_filtered_dev")
         return out.view(-1, 2)
+
+    @property
+    def radius_batch(self) -> int:
+        return int(_lib().sf21_hii_radius_batch(self._h))
 
     def profile(self, enable: bool = True):
         _check(_lib().sf21_hii_profile(self._h, int(enable)), "sf21_hii_profile")

@@ -234,4 +234,49 @@ __device__ __forceinline__ int fold_sq(int i, int n) {
   return f * f;
 }
 
+// Top-hat window W(x) = 3 j1(x)/x = 3 (sin x - x cos x)/x^3, x = |k| R
+// (auxiliary.c:350-359), in FP32 with a phase accurate to ~1e-7 absolute:
+//   - x = kappa sqrt(m), kappa = dk R as a float pair (hi, lo) rounded from
+//     FP64, m = integer |k|^2 in dk units: kept as a double-float p + x_lo
+//     (Dekker's exact product via fmaf; sqrt(m) completed by its exact
+//     residual m - s^2, so the hardware square root's last bit is
+//     irrelevant);
+//   - below x = 1.25: the series 3 sum_k (-1)^(k+1) 2k x^(2k-2)/(2k+1)! to
+//     x^12 (truncation < 1e-11), where the closed form cancels;
+//   - above: x - n pi/2 with a three-part Cody-Waite constant (12-bit first
+//     parts: n*C1 and n*C2 exact for n < 4096, i.e. x < 6400), then Taylor
+//     sin to r^9 and cos to r^10 on |r| <= pi/4 (truncation < 2e-9) and the
+//     quadrant n mod 4.
+// A plain FP32 x = kappa*sqrtf(m) carries ~x*6e-8 of phase error, i.e. a
+// relative W error of ~5e-5 at x ~ 700 (R = 100 Mpc at 512^3); the library
+// sincosf would also carry its large-argument reduction code.
+__device__ __forceinline__ float w_tophat_k(float2 kap, int m) {
+  const float fm = (float)m;
+  const float s = __builtin_amdgcn_sqrtf(fm);
+  const float s_lo = fmaf(-s, s, fm) * (0.5f * __builtin_amdgcn_rcpf(s));
+  const float p = kap.x * s;
+  const float x_lo = fmaf(kap.x, s, -p) + fmaf(kap.x, s_lo, kap.y * s);
+  const float x = p + x_lo;
+  const float q = x * x;
+  const float ser = 1.0f + q * (-0.1f + q * (1.0f / 280.0f + q * (-1.0f / 15120.0f +
+                    q * (1.0f / 1330560.0f + q * (-1.0f / 172972800.0f + q * (1.0f / 31135104000.0f))))));
+  const float nq = rintf(p * 0.6366197466850281f);
+  float r = fmaf(-nq, 1.5703125f, p);
+  r = fmaf(-nq, 4.837512969970703e-4f, r);
+  r = fmaf(-nq, 7.549790126404332e-8f, r);
+  r += x_lo;
+  const float r2 = r * r;
+  const float sr = r * (1.0f + r2 * (-1.0f / 6.0f + r2 * (1.0f / 120.0f + r2 * (-1.0f / 5040.0f +
+                   r2 * (1.0f / 362880.0f)))));
+  const float cr = 1.0f + r2 * (-0.5f + r2 * (1.0f / 24.0f + r2 * (-1.0f / 720.0f +
+                   r2 * (1.0f / 40320.0f + r2 * (-1.0f / 3628800.0f)))));
+  const int qd = (int)nq & 3;
+  const float sn = (qd & 1) ? cr : sr, cn = (qd & 1) ? sr : cr;
+  const float sx = (qd & 2) ? -sn : sn;                   // sin(r + qd pi/2)
+  const float cx = ((qd + 1) & 2) ? -cn : cn;             // cos(r + qd pi/2)
+  const float inv = __builtin_amdgcn_rcpf(x);
+  const float big = 3.0f * (sx - x * cx) * (inv * inv * inv);
+  return m == 0 ? 1.0f : (x < 1.25f ? ser : big);
+}
+
 }  // namespace sf21

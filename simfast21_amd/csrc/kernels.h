@@ -29,7 +29,7 @@ void launch_fwd_rows(int n, const float* a, const float* b, const float* c, Cell
 void launch_cols(int n, int width, int axis, int sign, const float2* in, float2* out,
                  int nbatch, long bstride, const float2* tw, hipStream_t s);
 void launch_filter_i(int n, const float2* spec, float2* u, int nrad, long ustride,
-                     const float* wlut, long m_lut, const float* kappa, const float2* tw,
+                     const float* wlut, long m_lut, const float2* kappa, const float2* tw,
                      hipStream_t s, int tile, int wmode);
 void launch_inv_rows_crit(int n, const float2* v, int nrad, long vstride, const int* d2max,
                           const float* bal, int* wmap, float2* dbg, hipStream_t s,
@@ -99,7 +99,8 @@ void launch_rows_seed(int n, float2* u, const float2* v, int nrad, long ustride,
 void launch_ball_i(int n, bool acc, const float2* in, long istride, float2* out, long ostride,
                    int nrad, const float* bt, const float2* tw, hipStream_t s);
 void launch_filter_acc_i(int n, const float2* in, long istride, int nrad, const float* wlut,
-                         long mlut, float2* acc, const float2* tw, hipStream_t s);
+                         long mlut, const float2* kappa, int wmode, float2* acc, const float2* tw,
+                         hipStream_t s);
 void launch_relaxed_state(long n3, const float2* cacc, const float* rec, const float* ion,
                           const float* delta1, float* xt, float* t, float* fq, float* fa,
                           double* part, double* st, hipStream_t s);

@@ -381,10 +381,10 @@ __global__ __launch_bounds__(256) void k_rows_seed(float2* u, const float2* v, i
 // Sweep 2, along i: forward of each seed line, times W_R, summed over the
 // batch, one inverse along i, added to acc ((p,j)-space).
 // ---------------------------------------------------------------------------
-template <int L, int TILE>
+template <int L, int TILE, bool WTAB>
 __global__ __launch_bounds__(TILE*(L / Plan<L>::E)) void k_filter_acc_i(
     const float2* in, long istride, int nrad, const float* __restrict__ wlut, long mlut,
-    float2* acc, const float2* __restrict__ tw) {
+    const float2* __restrict__ kappa, float2* acc, const float2* __restrict__ tw) {
   constexpr int E = Plan<L>::E, TPL = L / E, NT = L / TILE;
   __shared__ float lds[TILE * 2 * LdsLine<L>::kFloats];
   const int c = threadIdx.x % TILE, t = threadIdx.x / TILE;
@@ -405,12 +405,21 @@ __global__ __launch_bounds__(TILE*(L / Plan<L>::E)) void k_filter_acc_i(
 #pragma unroll
     for (int m = 0; m < E; ++m) v[m] = src[(tl + TPL * m) * (L * L)];
     line_fft<L, -1>(v, sre, sim, tl, tw);
-    const float* __restrict__ w = wlut + (long)r * mlut;
+    // the same W_R as the forward pass (k_filter_i): table or FP32 evaluation
+    float wv[E];
+    if constexpr (WTAB) {
+      const float* __restrict__ w = wlut + (long)r * mlut;
+#pragma unroll
+      for (int m = 0; m < E; ++m) wv[m] = w[mjp + fold_sq(tl + TPL * m, L)];
+    } else {
+      const float2 kr = kappa[r];
+#pragma unroll
+      for (int m = 0; m < E; ++m) wv[m] = w_tophat_k(kr, mjp + fold_sq(tl + TPL * m, L));
+    }
 #pragma unroll
     for (int m = 0; m < E; ++m) {
-      const float wv = w[mjp + fold_sq(tl + TPL * m, L)];
-      z[m].x += wv * v[m].x;
-      z[m].y += wv * v[m].y;
+      z[m].x += wv[m] * v[m].x;
+      z[m].y += wv[m] * v[m].y;
     }
   }
   line_fft<L, +1>(z, sre, sim, t, tw);
@@ -640,10 +649,16 @@ static void ball_i_t(const float2* in, long istride, float2* out, long ostride, 
 }
 template <int L>
 static void filter_acc_t(const float2* in, long istride, int nrad, const float* wlut, long mlut,
-                         float2* acc, const float2* tw, hipStream_t s) {
+                         const float2* kappa, int wmode, float2* acc, const float2* tw,
+                         hipStream_t s) {
   constexpr int TILE = 16, TPL = L / Plan<L>::E;
-  hipLaunchKernelGGL((k_filter_acc_i<L, TILE>), dim3((unsigned)((long)L * (L / TILE))),
-                     dim3(TILE * TPL), 0, s, in, istride, nrad, wlut, mlut, acc, tw);
+  const dim3 g((unsigned)((long)L * (L / TILE))), b(TILE * TPL);
+  if (wmode == 0)
+    hipLaunchKernelGGL((k_filter_acc_i<L, TILE, true>), g, b, 0, s, in, istride, nrad, wlut, mlut,
+                       kappa, acc, tw);
+  else
+    hipLaunchKernelGGL((k_filter_acc_i<L, TILE, false>), g, b, 0, s, in, istride, nrad, wlut, mlut,
+                       kappa, acc, tw);
 }
 
 #define SF21_DIFF_SIZES(X) X(32) X(64) X(96) X(128) X(192) X(256) X(512)
@@ -694,9 +709,10 @@ void launch_ball_i(int n, bool acc, const float2* in, long istride, float2* out,
   }
 }
 void launch_filter_acc_i(int n, const float2* in, long istride, int nrad, const float* wlut,
-                         long mlut, float2* acc, const float2* tw, hipStream_t s) {
+                         long mlut, const float2* kappa, int wmode, float2* acc, const float2* tw,
+                         hipStream_t s) {
   switch (n) {
-#define C(L) case L: filter_acc_t<L>(in, istride, nrad, wlut, mlut, acc, tw, s); break;
+#define C(L) case L: filter_acc_t<L>(in, istride, nrad, wlut, mlut, kappa, wmode, acc, tw, s); break;
     SF21_DIFF_SIZES(C)
 #undef C
   }

@@ -215,25 +215,10 @@ __global__ __launch_bounds__(TILE*(L / Plan<L>::E)) void k_cols(const float2* in
 // an FP64 table indexed by the integer |k|^2, rounded to float), inverse
 // along i.  The forward spectrum along i never leaves the registers.
 // ------------------------------------------------------------------------
-// W(x) = 3 j1(x)/x in FP32 (auxiliary.c:350-359 evaluated on the fly): the
-// Taylor series 3 sum_k (-1)^(k+1) 2k x^(2k-2)/(2k+1)! to x^12 below 1.25
-// (truncation < 1e-11), 3 (sin x - x cos x)/x^3 above, where the
-// subtraction no longer cancels.
-__device__ __forceinline__ float w_tophat(float x) {
-  if (x < 1.25f) {
-    const float q = x * x;
-    return 1.0f + q * (-0.1f + q * (1.0f / 280.0f + q * (-1.0f / 15120.0f +
-           q * (1.0f / 1330560.0f + q * (-1.0f / 172972800.0f + q * (1.0f / 31135104000.0f))))));
-  }
-  float sn, cs;
-  sincosf(x, &sn, &cs);
-  return 3.0f * (sn - x * cs) / (x * x * x);
-}
-
 template <int L, int TILE, bool WTAB>
 __global__ __launch_bounds__(TILE*(L / Plan<L>::E)) void k_filter_i(
     const float2* __restrict__ spec, float2* __restrict__ u, int nrad, long ustride,
-    const float* __restrict__ wlut, long mlut, const float* __restrict__ kappa,
+    const float* __restrict__ wlut, long mlut, const float2* __restrict__ kappa,
     const float2* __restrict__ tw) {
   constexpr int E = Plan<L>::E, TPL = L / E, NT = L / TILE;
   __shared__ float lds[TILE * 2 * LdsLine<L>::kFloats];
@@ -261,10 +246,10 @@ __global__ __launch_bounds__(TILE*(L / Plan<L>::E)) void k_filter_i(
         v[m] = make_float2(z[m].x * wv, z[m].y * wv);
       }
     } else {
-      const float kr = kappa[r];  // dk * R
+      const float2 kr = kappa[r];  // dk * R as hi + lo
 #pragma unroll
       for (int m = 0; m < E; ++m) {
-        const float wv = w_tophat(kr * sqrtf((float)(mjp + fold_sq(tl + TPL * m, L))));
+        const float wv = w_tophat_k(kr, mjp + fold_sq(tl + TPL * m, L));
         v[m] = make_float2(z[m].x * wv, z[m].y * wv);
       }
     }
@@ -613,7 +598,7 @@ void launch_cols(int n, int width, int axis, int sign, const float2* in, float2*
 }
 
 void launch_filter_i(int n, const float2* spec, float2* u, int nrad, long ustride,
-                     const float* wlut, long m_lut, const float* kappa, const float2* tw,
+                     const float* wlut, long m_lut, const float2* kappa, const float2* tw,
                      hipStream_t s, int tile, int wmode) {
 #define X(L, TL)                                                                          \
   if (n == L && tile == TL) {                                                             \

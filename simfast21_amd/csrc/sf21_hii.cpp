@@ -75,8 +75,8 @@ struct sf21_hii_ctx {
   std::vector<int> d2;
   int n = 0, h = 0, K = 0;
   int tile_filter = 16;
-  int filter_w = 0;  // 0: FP64 table per |k|^2, 1: FP32 on the fly
-  float* d_kappa = nullptr;
+  int filter_w = 1;  // 1: FP32 on the fly, 0: FP64 table per |k|^2
+  float2* d_kappa = nullptr;  // dk R per radius as a float pair (hi, lo)
   long n3 = 0, nh = 0;
   double L = 0, dx = 0, dk = 0;
   hipStream_t stream = nullptr;
@@ -518,7 +518,7 @@ void enqueue_diff_backward(sf21_hii_ctx* c, const float* g, const float* emis, d
     const int kp = (kk + 1) / 2;
     launch(c, "filter_i", [&] {
       launch_filter_i(n, c->spec, c->ubuf, kk, n3, c->wlut + (long)rb * c->mlut, c->mlut,
-                      c->d_kappa + rb, c->tw, s, c->tile_filter, 0);
+                      c->d_kappa + rb, c->tw, s, c->tile_filter, c->filter_w);
     });
     launch(c, "cols_inv_j", [&] { launch_cols(n, n, 0, +1, c->ubuf, c->ubuf, kk, n3, c->tw, s); });
     launch(c, "rows_phi", [&] { launch_rows_phi(n, c->ubuf, kk, n3, c->bal, c->df_st, inv_tau, c->tw, s); });
@@ -562,7 +562,7 @@ void enqueue_diff_backward(sf21_hii_ctx* c, const float* g, const float* emis, d
     launch(c, "cols_inv_j", [&] { launch_cols(n, n, 0, +1, c->df_v, c->df_v, kp, n3, c->tw, s); });
     launch(c, "filter_i", [&] {
       launch_filter_i(n, c->spec, c->ubuf, kk, n3, c->wlut + (long)rb * c->mlut, c->mlut,
-                      c->d_kappa + rb, c->tw, s, c->tile_filter, 0);
+                      c->d_kappa + rb, c->tw, s, c->tile_filter, c->filter_w);
     });
     launch(c, "cols_inv_j", [&] { launch_cols(n, n, 0, +1, c->ubuf, c->ubuf, kk, n3, c->tw, s); });
     launch(c, "rows_seed", [&] {
@@ -570,7 +570,8 @@ void enqueue_diff_backward(sf21_hii_ctx* c, const float* g, const float* emis, d
     });
     launch(c, "cols_fwd_j", [&] { launch_cols(n, n, 0, -1, c->ubuf, c->ubuf, kk, n3, c->tw, s); });
     launch(c, "filter_acc_i", [&] {
-      launch_filter_acc_i(n, c->ubuf, n3, kk, c->wlut + (long)rb * c->mlut, c->mlut, c->df_g, c->tw, s);
+      launch_filter_acc_i(n, c->ubuf, n3, kk, c->wlut + (long)rb * c->mlut, c->mlut, c->d_kappa + rb,
+                          c->filter_w, c->df_g, c->tw, s);
     });
   }
   launch(c, "cols_inv_j", [&] { launch_cols(n, n, 0, +1, c->df_g, c->df_g, 1, 0, c->tw, s); });
@@ -636,10 +637,10 @@ int sf21_hii_create(const sf21_hii_config* cfg, sf21_hii_ctx** out) {
   while ((long)(c->h + 1) * (c->h + 1) <= wmax) c->h++;
   while ((long)c->h * c->h > wmax) c->h--;
   if (c->h > c->n / 2) c->h = c->n / 2;
-  c->K = cfg->radius_batch > 0 ? cfg->radius_batch : 8;
+  c->K = cfg->radius_batch > 0 ? cfg->radius_batch : 16;
   if (cfg->filter_tile == 4 || cfg->filter_tile == 8 || cfg->filter_tile == 16)
     c->tile_filter = cfg->filter_tile;
-  c->filter_w = cfg->filter_w == 1 ? 1 : 0;
+  c->filter_w = cfg->filter_w == 2 ? 0 : 1;
   if (c->K > (int)c->radii.size()) c->K = (int)c->radii.size();
 
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -661,10 +662,14 @@ int sf21_hii_create(const sf21_hii_config* cfg, sf21_hii_ctx** out) {
       rc = SF21_ERR_HIP;
       goto fail;
     }
-    std::vector<float> kap(c->radii.size());
-    for (size_t r = 0; r < c->radii.size(); ++r) kap[r] = (float)(c->dk * c->radii[r]);
+    std::vector<float2> kap(c->radii.size());
+    for (size_t r = 0; r < c->radii.size(); ++r) {
+      const double k = c->dk * c->radii[r];
+      const float hi = (float)k;
+      kap[r] = make_float2(hi, (float)(k - (double)hi));
+    }
     if ((rc = dev_alloc(&c->d_kappa, kap.size()))) goto fail;
-    if (hipMemcpy(c->d_kappa, kap.data(), sizeof(float) * kap.size(), hipMemcpyHostToDevice) !=
+    if (hipMemcpy(c->d_kappa, kap.data(), sizeof(float2) * kap.size(), hipMemcpyHostToDevice) !=
         hipSuccess) {
       rc = SF21_ERR_HIP;
       goto fail;
@@ -692,8 +697,14 @@ int sf21_hii_create(const sf21_hii_config* cfg, sf21_hii_ctx** out) {
     c->dt.n = c->n;
     c->dt.h = c->h;
     c->npart = dt_partials_count(c->n);
-    if ((rc = dev_alloc(&c->spec, (size_t)c->n3)) || (rc = dev_alloc(&c->ubuf, (size_t)c->n3 * c->K)) ||
-        (rc = dev_alloc(&c->wmap, (size_t)c->n3)) || (rc = dev_alloc(&c->gbuf, (size_t)c->n3)) ||
+    // the plain path uses the batch buffer only for the inspection hook
+    if (cfg->path == SF21_PATH_PLAIN) c->K = 1;
+    if ((rc = dev_alloc(&c->spec, (size_t)c->n3))) goto fail;
+    // automatic batch: halve K until the K n^3 complex buffer fits
+    while ((rc = dev_alloc(&c->ubuf, (size_t)c->n3 * c->K)) && cfg->radius_batch <= 0 && c->K > 1)
+      c->K /= 2;
+    if (rc) goto fail;
+    if ((rc = dev_alloc(&c->wmap, (size_t)c->n3)) || (rc = dev_alloc(&c->gbuf, (size_t)c->n3)) ||
         (rc = dev_alloc(&c->bal_part, (size_t)2 * balance_partials_count(c->n3))) ||
         (rc = dev_alloc(&c->bal, 1)) ||
         (rc = dev_alloc(&c->partials, (size_t)std::max(c->npart, clamp_partials_count(c->n3)))))
@@ -868,6 +879,8 @@ int sf21_hii_filtered_dev(sf21_hii_ctx* c, const float* a, const float* b, const
   if (c->prof) collect(c);
   return check_hip();
 }
+
+int sf21_hii_radius_batch(const sf21_hii_ctx* c) { return c ? c->K : SF21_ERR_ARG; }
 
 int sf21_hii_profile(sf21_hii_ctx* c, int enable) {
   if (!c) return SF21_ERR_ARG;

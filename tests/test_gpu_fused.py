@@ -35,9 +35,10 @@ def near_threshold(rec64, ion64, err_rec, err_ion, rtol=1e-4):
     return a | b
 
 
-def run_case(torch, case, path=sf.PATH_FUSED, radius_batch=0):
+def run_case(torch, case, path=sf.PATH_FUSED, radius_batch=0, filter_w=0):
     n, side, radii = case["n"], case["side"], case["radii"]
-    ctx = sf.HiiContext(n, side, radii, sf.INPUT_DIRECT, path, radius_batch=radius_batch)
+    ctx = sf.HiiContext(n, side, radii, sf.INPUT_DIRECT, path, radius_batch=radius_batch,
+                        filter_w=filter_w)
     a = torch.from_numpy(case["rec"].ravel()).cuda()
     b = torch.from_numpy(case["ion"].ravel()).cuda()
     x, neu = ctx.run_slice(a, b)
@@ -47,12 +48,15 @@ def run_case(torch, case, path=sf.PATH_FUSED, radius_batch=0):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("n,zeta", [(64, 0.3), (64, 3.0), (128, 1.0)])
-def test_fused_vs_oracle(torch_cuda, n, zeta):
+@pytest.mark.parametrize("n,zeta,filter_w", [(64, 0.3, 0), (64, 3.0, 0), (128, 1.0, 0),
+                                             (64, 3.0, 2), (128, 1.0, 2)])
+def test_fused_vs_oracle(torch_cuda, n, zeta, filter_w):
+    """filter_w 0: top-hat weights evaluated in FP32 in the kernel (the
+    default); 2: the FP64 table per integer |k|^2."""
     torch = torch_cuda
     case = direct_case(n, zeta)
     side, radii, dx = case["side"], case["radii"], case["side"] / n
-    x_g, neu_g, w_g, _ = run_case(torch, case)
+    x_g, neu_g, w_g, _ = run_case(torch, case, filter_w=filter_w)
     x_o, neu_o, fc_o = orc.hii_slice(n, side, radii, case["rec"], case["ion"], precision=64, dilation=1)
     # 1) exact dilation of the GPU's own map + quick fill
     dil = orc.dilate_brute(n, w_g).astype(bool)
@@ -61,7 +65,7 @@ def test_fused_vs_oracle(torch_cuda, n, zeta):
     # 2) centre-map differences are near-threshold cells only
     w_o = weights_from_first_centre(fc_o, radii, dx)
     diff = np.nonzero(w_g != w_o)[0]
-    print(f"n={n} zeta={zeta}: centre-map mismatches {diff.size} of {n**3}, "
+    print(f"n={n} zeta={zeta} filter_w={filter_w}: centre-map mismatches {diff.size} of {n**3}, "
           f"mean x_HII gpu {1 - neu_g:.7f} oracle {1 - neu_o:.7f}")
     if diff.size:
         d2 = np.array([orc.d2max(R, dx) for R in radii])
@@ -73,7 +77,7 @@ def test_fused_vs_oracle(torch_cuda, n, zeta):
             cand = [k for k in range(len(radii)) if (w_g[cell] >= d2[k]) != (w_o[cell] >= d2[k])]
             k0 = min(cand)
             groups.setdefault(int(d2[k0]), []).append(cell)
-        ctx = sf.HiiContext(n, side, radii, sf.INPUT_DIRECT, sf.PATH_FUSED)
+        ctx = sf.HiiContext(n, side, radii, sf.INPUT_DIRECT, sf.PATH_FUSED, filter_w=filter_w)
         a = torch.from_numpy(case["rec"].ravel()).cuda()
         b = torch.from_numpy(case["ion"].ravel()).cuda()
         for dval, cells in sorted(groups.items()):
