@@ -83,6 +83,20 @@ def make_inputs(torch, n, side, n_slices, n_src, seed=3001, slope=-2.5, sigma=1.
     return recs, ions
 
 
+def measured_traffic(kernel, n, n_radii, radius_batch):
+    """HBM bytes per launch of a kernel as measured by the rocprofv3 counter
+    runs recorded under profiles/ (null when the workload differs)."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "hbm_traffic.json")) as f:
+            rec = json.load(f)
+    except OSError:
+        return None
+    w = rec.get("workload", {})
+    if (w.get("n"), w.get("radii"), w.get("radius_batch")) != (n, n_radii, radius_batch):
+        return None
+    return rec.get("bytes_per_launch", {}).get(kernel)
+
+
 def bench_diff(torch, sf, n, side, radii, n_src, taus=(0.05, 0.02, 0.01), reps=3, seed=5005):
     """Differentiable mode at one redshift: Gaussian g (sigma 1.2, P ~ k^-2.5),
     emissivity pattern from n_src sources ~ (1+delta)^2, zeta set for
@@ -270,8 +284,9 @@ def main():
     if dom in algo:
         ach = algo[dom] / (avg_ms * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
-                "avg_launch_ms": round(avg_ms, 4)}
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom, n, len(radii), K),
+                "kernel": dom, "avg_launch_ms": round(avg_ms, 4),
+                "algorithmic_bytes": algo[dom]}
     # whole-step algorithmic roofline (per voxel-radius 80h+4 B, per voxel-slice 80h+25 B)
     hfac = (n // 2 + 1) / n
     step_bytes = S * v * (len(radii) * (80 * hfac + 4) + (80 * hfac + 25))
