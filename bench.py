@@ -97,6 +97,68 @@ def measured_traffic(kernel, n, n_radii, radius_batch):
     return rec.get("bytes_per_launch", {}).get(kernel)
 
 
+def bench_uneven(torch, sf, synth, n_boxes=64, steps=3, seed=4000):
+    """Parameter-sweep batch: n_boxes boxes with N drawn uniformly from
+    {64, 96, 128, 192, 256} at a fixed cell size dx = 300/256 Mpc (sides
+    75-300 Mpc), P(k) slope ~ U[-2.8, -2.2], sigma 1.2, 4000 (N/128)^3
+    sources, efficiency zeta ~ 1.5 lognormal(0, 0.5); radius ladder per size
+    from R_max = side/4 down by 0.9 to the cell size.  All boxes in one
+    hipGraph launch (sf21_hii_run_multi_dev); per-size throughput from each
+    box run alone."""
+    rng = np.random.default_rng(seed)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed)
+    dx = 300.0 / 256
+    sizes = [int(v) for v in rng.choice([64, 96, 128, 192, 256], n_boxes)]
+    ctxs, ladders, a, b = {}, {}, [], []
+    for n in sorted(set(sizes)):
+        side = n * dx
+        ladders[n] = synth.geometric_ladder(side / 4, 0.9, dx)
+        ctxs[n] = sf.HiiContext(n, side, ladders[n], sf.INPUT_DIRECT, sf.PATH_FUSED)
+    for n in sizes:
+        slope = float(rng.uniform(-2.8, -2.2))
+        zeta = 1.5 * float(np.exp(0.5 * rng.standard_normal()))
+        g = gaussian_field_dev(torch, gen, n, slope, 1.2)
+        e = torch.exp(g.double())
+        e /= e.mean()
+        b.append((sources_dev(torch, gen, e, int(round(4000 * (n / 128) ** 3))) * zeta).contiguous())
+        a.append((e ** 2).float().reshape(-1).contiguous())
+        del g, e
+    boxes = [ctxs[n] for n in sizes]
+    outs = [torch.empty_like(x) for x in a]
+    sf.run_multi(boxes, a, b, out_list=outs)  # capture
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        _, neu = sf.run_multi(boxes, a, b, out_list=outs)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    units = sum(n ** 3 * len(ladders[n]) for n in sizes)
+    per = {}
+    for n in sorted(set(sizes)):
+        idx = [k for k, v in enumerate(sizes) if v == n]
+        boxes[idx[0]].run_slice(a[idx[0]], b[idx[0]], xhii=outs[idx[0]])
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for k in idx:
+            boxes[k].run_slice(a[k], b[k], xhii=outs[k])
+        torch.cuda.synchronize()
+        mb = (time.perf_counter() - t1) * 1e3 / len(idx)
+        per[str(n)] = {"boxes": len(idx), "radii": len(ladders[n]), "ms_per_box_alone": round(mb, 3),
+                       "voxel_radius_per_s_alone": round(n ** 3 * len(ladders[n]) / (mb * 1e-3), 1)}
+    for c in ctxs.values():
+        c.close()
+    return {
+        "workload": (f"{n_boxes} boxes, N ~ U{{64,96,128,192,256}} at dx {dx:g} Mpc, slope ~ U[-2.8,-2.2], "
+                     f"zeta ~ 1.5 lognormal(0.5), ladder side/4 x 0.9^k to dx; one hipGraph launch"),
+        "voxel_radius_total": units,
+        "ms_per_launch": round(ms, 3),
+        "voxel_radius_per_s": round(units / (ms * 1e-3), 1),
+        "mean_xhii_range": [round(1 - max(neu), 4), round(1 - min(neu), 4)],
+        "per_size": per,
+    }
+
+
 def bench_diff(torch, sf, n, side, radii, n_src, taus=(0.05, 0.02, 0.01), reps=3, seed=5005):
     """Differentiable mode at one redshift: Gaussian g (sigma 1.2, P ~ k^-2.5),
     emissivity pattern from n_src sources ~ (1+delta)^2, zeta set for
@@ -204,6 +266,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--radius-batch", type=int, default=0)
     ap.add_argument("--no-diff", action="store_true")
+    ap.add_argument("--no-uneven", action="store_true")
+    ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one 1 GiB device copy (torch copy_) as the HBM-counter reference")
     args = ap.parse_args()
@@ -239,6 +303,12 @@ def main():
     ctx = sf.HiiContext(n, side, radii, sf.INPUT_DIRECT, sf.PATH_FUSED,
                         radius_batch=args.radius_batch)
     torch.cuda.synchronize()
+    tuned = None
+    if args.radius_batch <= 0 and not args.no_tune:
+        # launch choices picked by the autotuner on one slice (not timed)
+        tune_ms = ctx.autotune(recs[S // 2], ions[S // 2])
+        tuned = dict(ctx.tuning(), tune_ms_per_slice=round(tune_ms, 3))
+        print(f"[bench] autotuned: {tuned}", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         _, neu = ctx.run_batch(recs, ions, out_list=outs)
     torch.cuda.synchronize()
@@ -293,10 +363,11 @@ def main():
     step_roof = {"bytes": step_bytes, "achieved_GBs": round(step_bytes / (ms * 1e-3) / 1e9, 1),
                  "frac": round(step_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
 
+    ctx.close()
+    del ctx
+    torch.cuda.empty_cache()
     plain = None
     if not args.no_plain:
-        del ctx
-        torch.cuda.empty_cache()
         pctx = sf.HiiContext(n, side, radii, sf.INPUT_DIRECT, sf.PATH_PLAIN)
         pout = torch.empty_like(recs[0])
         pctx.run_slice(recs[S // 2], ions[S // 2], xhii=pout)
@@ -310,6 +381,15 @@ def main():
                  "mean_xhii_slice": round(1 - pneu, 6), "fused_mean_xhii_slice": round(1 - neu[S // 2], 6)}
         print(f"[bench] plain {pms:.1f} ms/slice", file=sys.stderr, flush=True)
         del pctx
+
+    uneven = None
+    if not args.no_uneven:
+        try:
+            uneven = bench_uneven(torch, sf, synth)
+        finally:
+            torch.cuda.empty_cache()
+        print(f"[bench] uneven batch: {uneven['ms_per_launch']:.1f} ms per launch, "
+              f"{uneven['voxel_radius_per_s']:.3g} voxel-radius/s", file=sys.stderr, flush=True)
 
     diff = None
     if not args.no_diff:
@@ -344,13 +424,14 @@ def main():
             "workload": (f"batched exact bubble finder, {n}^3 float32, side {side:g} Mpc, {S} slices, "
                          f"{len(radii)} radii ({100.0 * n / 512:g} Mpc x 0.9^k to the cell size), "
                          f"{n_src} sources/slice"),
-            "n": n, "slices": S, "radii": len(radii), "radius_batch": K,
+            "n": n, "slices": S, "radii": len(radii), "radius_batch": K, "tuning": tuned,
         },
         "roofline": roof,
         "step_roofline": step_roof,
         "profiled_ms_per_step": round(prof_ms, 3),
         "kernels_ms": {k: [c, round(t, 3)] for k, (c, t) in stats.items()},
         "plain": plain,
+        "uneven_batch": uneven,
         "diff": diff,
         "cpu_baseline": cpu,
     }

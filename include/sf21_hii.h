@@ -98,6 +98,17 @@ int sf21_hii_run_batch_dev(sf21_hii_ctx *ctx, int n_slices, const float *const *
                            const float *const *b, const float *const *c, const double *z,
                            float *const *xhii, double *neutral);
 
+/* Boxes of different sizes in one submission (device pointers): box i runs
+ * on context ctx[i] (fused path, one device; boxes may share a context of
+ * their size and ladder, e.g. one per distinct N).  The whole set is
+ * captured into one hipGraph on ctx[0]'s stream, cached on ctx[0] by
+ * (contexts, pointers, redshifts), and replayed.  neutral: host array of
+ * nbox doubles.  No counterpart in the reference (one box per run there);
+ * each box is get_HIIbubbles.c:216-479 on its own inputs. */
+int sf21_hii_run_multi_dev(int nbox, sf21_hii_ctx *const *ctx, const float *const *a,
+                           const float *const *b, const float *const *c, const double *z,
+                           float *const *xhii, double *neutral);
+
 /* Halo mode.  halos: n_halos records {float Mass, x, y, z} with positions
  * in N_halo cell units (auxiliary.h:13-17, output_files.txt:10-14).
  * Emissivity box = smooth_box(CIC(Rion(M, z))) * (dx_s/dx_h)^3 * fesc
@@ -155,6 +166,23 @@ int sf21_hii_weight_map(sf21_hii_ctx *ctx, int *host_out);
  * get_HIIbubbles.c:324-325 gives them. */
 int sf21_hii_filtered_dev(sf21_hii_ctx *ctx, const float *a, const float *b, const float *c,
                           double z, int radius_index, float *out);
+
+/* Autotuner (fused path; no counterpart in the reference, whose FFTW plans
+ * are tuned by FFTW_MEASURE at get_HIIbubbles.c:110-183).  Times the slice
+ * (a, b, c, z; device pointers) for every candidate -- filter tile 16/8,
+ * radius batch 8/12/16 (capped by the ladder; skipped when it does not fit),
+ * top-hat weights in the kernel or from the FP64 table, cols tile 16/8 --
+ * keeps the fastest in the context and appends it to the cache file
+ * (cache_path, else $SF21_TUNE_CACHE, else not recorded) keyed by
+ * (device arch, n, number of radii).  A context created with radius_batch,
+ * filter_tile and filter_w all 0 takes the cached choice for its key from
+ * $SF21_TUNE_CACHE.  *best_ms (nullable): the winner's time per slice. */
+int sf21_hii_autotune(sf21_hii_ctx *ctx, const float *a, const float *b, const float *c,
+                      double z, const char *cache_path, double *best_ms);
+/* Current choices: filter tile, radius batch, top-hat weights (1 in the
+ * kernel, 2 table), cols tile; any pointer may be NULL. */
+int sf21_hii_tuning(const sf21_hii_ctx *ctx, int *filter_tile, int *radius_batch, int *filter_w,
+                    int *cols_tile);
 
 /* Effective radii per fused filter pass (the automatic choice depends on
  * free device memory); negative on error. */

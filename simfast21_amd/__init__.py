@@ -110,9 +110,12 @@ def _lib():
     L.sf21_hii_run_slice_dev.argtypes = [vp, vp, vp, vp, d, vp, dp]
     L.sf21_hii_run_batch_dev.argtypes = [vp, i, vp, vp, vp, dp, vp, dp]
     L.sf21_hii_weight_map.argtypes = [vp, vp]
+    L.sf21_hii_run_multi_dev.argtypes = [i, vp, vp, vp, vp, dp, vp, dp]
     L.sf21_hii_filtered_dev.argtypes = [vp, vp, vp, vp, d, i, vp]
     L.sf21_hii_profile.argtypes = [vp, i]
     L.sf21_hii_radius_batch.argtypes = [vp]
+    L.sf21_hii_autotune.argtypes = [vp, vp, vp, vp, d, ctypes.c_char_p, dp]
+    L.sf21_hii_tuning.argtypes = [vp, ip, ip, ip, ip]
     L.sf21_hii_kernel_stats.argtypes = [vp, i, ctypes.c_char_p, i, ctypes.POINTER(ctypes.c_long), dp]
     L.sf21_hii_reset_stats.argtypes = [vp]
     L.sf21_hii_power_spectrum_dev.argtypes = [vp, vp, i, vp, vp, vp]
@@ -321,6 +324,23 @@ class HiiContext:
                                             int(radius_index), _ptr(out)), "sf21_hii_filtered_dev")
         return out.view(-1, 2)
 
+    def autotune(self, a, b, c=None, z: float = 0.0, cache_path: str | None = None) -> float:
+        """Time the fused path's candidates on this slice, keep the fastest;
+        returns its ms per slice."""
+        import torch
+        torch.cuda.current_stream().synchronize()
+        ms = ctypes.c_double()
+        _check(_lib().sf21_hii_autotune(self._h, _ptr(a), _ptr(b), _ptr(c), float(z),
+                                        None if cache_path is None else str(cache_path).encode(),
+                                        ctypes.byref(ms)), "sf21_hii_autotune")
+        return ms.value
+
+    def tuning(self) -> dict:
+        v = [ctypes.c_int() for _ in range(4)]
+        _check(_lib().sf21_hii_tuning(self._h, *[ctypes.byref(x) for x in v]), "sf21_hii_tuning")
+        return {"filter_tile": v[0].value, "radius_batch": v[1].value, "filter_w": v[2].value,
+                "cols_tile": v[3].value}
+
     @property
     def radius_batch(self) -> int:
         return int(_lib().sf21_hii_radius_batch(self._h))
@@ -343,6 +363,27 @@ class HiiContext:
                    "sf21_hii_kernel_stats")
             out[name.value.decode()] = (cnt.value, ms.value)
         return out
+
+
+def run_multi(contexts, a_list, b_list, c_list=None, z_list=None, out_list=None):
+    """Boxes of different sizes in one hipGraph launch (sf21_hii_run_multi_dev):
+    box i runs on contexts[i] (a HiiContext of its size).  Returns (outs,
+    neutral list)."""
+    import torch
+    nb = len(a_list)
+    if out_list is None:
+        out_list = [torch.empty_like(a) for a in a_list]
+    torch.cuda.current_stream().synchronize()
+    P = ctypes.c_void_p * nb
+    pc = P(*[c._h.value for c in contexts])
+    pa = P(*[_ptr(t) for t in a_list])
+    pb = P(*[_ptr(t) for t in b_list])
+    pcc = P(*[_ptr(t) for t in c_list]) if c_list is not None else None
+    po = P(*[_ptr(t) for t in out_list])
+    zs = (ctypes.c_double * nb)(*([0.0] * nb if z_list is None else z_list))
+    neu = (ctypes.c_double * nb)()
+    _check(_lib().sf21_hii_run_multi_dev(nb, pc, pa, pb, pcc, zs, po, neu), "sf21_hii_run_multi_dev")
+    return out_list, list(neu)
 
 
 class FftPlan:

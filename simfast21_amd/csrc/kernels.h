@@ -27,7 +27,7 @@ void launch_balance(long n3, const float* a, const float* b, const float* c, Cel
 void launch_fwd_rows(int n, const float* a, const float* b, const float* c, CellConsts k,
                      const float* bal, float2* y, const float2* tw, hipStream_t s);
 void launch_cols(int n, int width, int axis, int sign, const float2* in, float2* out,
-                 int nbatch, long bstride, const float2* tw, hipStream_t s);
+                 int nbatch, long bstride, const float2* tw, hipStream_t s, int tile = 16);
 void launch_filter_i(int n, const float2* spec, float2* u, int nrad, long ustride,
                      const float* wlut, long m_lut, const float2* kappa, const float2* tw,
                      hipStream_t s, int tile, int wmode);

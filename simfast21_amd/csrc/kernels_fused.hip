@@ -579,18 +579,26 @@ void launch_fwd_rows(int n, const float* a, const float* b, const float* c, Cell
 #undef X
 }
 
-constexpr int kTile = 16;
+template <int L, int TILE>
+static void cols_t(int width, int axis, int sign, const float2* in, float2* out, int nbatch,
+                   long bstride, const float2* tw, hipStream_t s) {
+  constexpr int NT = TILE * (L / Plan<L>::E);
+  dim3 g(L * ((width + TILE - 1) / TILE), nbatch);
+  if (sign < 0)
+    hipLaunchKernelGGL((k_cols<L, -1, TILE>), g, dim3(NT), 0, s, in, out, width, axis, bstride, tw);
+  else
+    hipLaunchKernelGGL((k_cols<L, +1, TILE>), g, dim3(NT), 0, s, in, out, width, axis, bstride, tw);
+}
 
+// tile: columns per block, 16 (default) or 8
 void launch_cols(int n, int width, int axis, int sign, const float2* in, float2* out,
-                 int nbatch, long bstride, const float2* tw, hipStream_t s) {
+                 int nbatch, long bstride, const float2* tw, hipStream_t s, int tile) {
 #define X(L)                                                                              \
   if (n == L) {                                                                           \
-    constexpr int NT = kTile * (L / Plan<L>::E);                                          \
-    dim3 g(L * ((width + kTile - 1) / kTile), nbatch);                                    \
-    if (sign < 0)                                                                         \
-      hipLaunchKernelGGL((k_cols<L, -1, kTile>), g, dim3(NT), 0, s, in, out, width, axis, bstride, tw); \
+    if (tile == 8)                                                                        \
+      cols_t<L, 8>(width, axis, sign, in, out, nbatch, bstride, tw, s);                   \
     else                                                                                  \
-      hipLaunchKernelGGL((k_cols<L, +1, kTile>), g, dim3(NT), 0, s, in, out, width, axis, bstride, tw); \
+      cols_t<L, 16>(width, axis, sign, in, out, nbatch, bstride, tw, s);                  \
     return;                                                                               \
   }
   SF21_SIZES(X)

@@ -2,8 +2,10 @@
 // finder.  See include/sf21_hii.h for the contract.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -75,6 +77,7 @@ struct sf21_hii_ctx {
   std::vector<int> d2;
   int n = 0, h = 0, K = 0;
   int tile_filter = 16;
+  int cols_tile = 16;  // columns per block of the per-radius inverse along j
   int filter_w = 1;  // 1: FP32 on the fly, 0: FP64 table per |k|^2
   float2* d_kappa = nullptr;  // dk R per radius as a float pair (hi, lo)
   long n3 = 0, nh = 0;
@@ -266,7 +269,7 @@ void enqueue_fused(sf21_hii_ctx* c, const float* a, const float* b, const float*
                       c->d_kappa + rb, c->tw, s, c->tile_filter, c->filter_w);
     });
     launch(c, "cols_inv_j",
-           [&] { launch_cols(n, n, 0, +1, c->ubuf, c->ubuf, kk, n3, c->tw, s); });
+           [&] { launch_cols(n, n, 0, +1, c->ubuf, c->ubuf, kk, n3, c->tw, s, c->cols_tile); });
     launch(c, "inv_rows_crit", [&] {
       launch_inv_rows_crit(n, c->ubuf, kk, n3, c->d_d2 + rb, c->bal, c->wmap, dbg, s, c->tw);
     });
@@ -520,7 +523,7 @@ void enqueue_diff_backward(sf21_hii_ctx* c, const float* g, const float* emis, d
       launch_filter_i(n, c->spec, c->ubuf, kk, n3, c->wlut + (long)rb * c->mlut, c->mlut,
                       c->d_kappa + rb, c->tw, s, c->tile_filter, c->filter_w);
     });
-    launch(c, "cols_inv_j", [&] { launch_cols(n, n, 0, +1, c->ubuf, c->ubuf, kk, n3, c->tw, s); });
+    launch(c, "cols_inv_j", [&] { launch_cols(n, n, 0, +1, c->ubuf, c->ubuf, kk, n3, c->tw, s, c->cols_tile); });
     launch(c, "rows_phi", [&] { launch_rows_phi(n, c->ubuf, kk, n3, c->bal, c->df_st, inv_tau, c->tw, s); });
     launch(c, "cols_fwd_j", [&] { launch_cols(n, n, 0, -1, c->ubuf, c->ubuf, kp, n3, c->tw, s); });
     launch(c, "ball_acc_i", [&] {
@@ -564,7 +567,7 @@ void enqueue_diff_backward(sf21_hii_ctx* c, const float* g, const float* emis, d
       launch_filter_i(n, c->spec, c->ubuf, kk, n3, c->wlut + (long)rb * c->mlut, c->mlut,
                       c->d_kappa + rb, c->tw, s, c->tile_filter, c->filter_w);
     });
-    launch(c, "cols_inv_j", [&] { launch_cols(n, n, 0, +1, c->ubuf, c->ubuf, kk, n3, c->tw, s); });
+    launch(c, "cols_inv_j", [&] { launch_cols(n, n, 0, +1, c->ubuf, c->ubuf, kk, n3, c->tw, s, c->cols_tile); });
     launch(c, "rows_seed", [&] {
       launch_rows_seed(n, c->ubuf, c->df_v, kk, n3, n3, c->bal, c->df_st, inv_tau, c->df_rho, c->tw, s);
     });
@@ -593,6 +596,96 @@ int diff_check(sf21_hii_ctx* c, const float* g, const float* emis, int nbins) {
   int rc;
   if ((rc = ensure_pk(c, nbins))) return rc;
   return ensure_diff(c);
+}
+
+// ---- autotuner: choices of the fused path, persisted per (arch, n, radii) ----
+struct TuneChoice {
+  int tile = 16, K = 16, w = 1, ct = 16;
+};
+
+std::string tune_key(const sf21_hii_ctx* c) {
+  hipDeviceProp_t prop;
+  std::string arch = "unknown";
+  if (hipGetDeviceProperties(&prop, c->cfg.device) == hipSuccess) {
+    arch = prop.gcnArchName;
+    arch = arch.substr(0, arch.find(':'));
+  }
+  char buf[256];
+  std::snprintf(buf, sizeof(buf), "%s n=%d radii=%zu", arch.c_str(), c->n, c->radii.size());
+  return buf;
+}
+
+const char* tune_path(const char* given) {
+  if (given && *given) return given;
+  const char* e = std::getenv("SF21_TUNE_CACHE");
+  return (e && *e) ? e : nullptr;
+}
+
+// Cache lines: "<key> | tile=T K=K w=W ct=C | <ms> ms"; the last match wins.
+bool tune_lookup(const char* path, const std::string& key, TuneChoice* out) {
+  FILE* f = path ? std::fopen(path, "r") : nullptr;
+  if (!f) return false;
+  char line[512];
+  bool found = false;
+  while (std::fgets(line, sizeof(line), f)) {
+    const char* bar = std::strchr(line, '|');
+    if (!bar) continue;
+    std::string k(line, (size_t)(bar - line));
+    while (!k.empty() && k.back() == ' ') k.pop_back();
+    TuneChoice t;
+    if (k == key && std::sscanf(bar + 1, " tile=%d K=%d w=%d ct=%d", &t.tile, &t.K, &t.w, &t.ct) == 4) {
+      *out = t;
+      found = true;
+    }
+  }
+  std::fclose(f);
+  return found;
+}
+
+void tune_store(const char* path, const std::string& key, const TuneChoice& t, double ms) {
+  FILE* f = path ? std::fopen(path, "a") : nullptr;
+  if (!f) return;
+  std::fprintf(f, "%s | tile=%d K=%d w=%d ct=%d | %.3f ms\n", key.c_str(), t.tile, t.K, t.w, t.ct, ms);
+  std::fclose(f);
+}
+
+void drop_graphs(sf21_hii_ctx* c) {
+  for (auto& g : c->graphs)
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  c->graphs.clear();
+}
+
+// Change the radius batch: the K n^3 pair buffer (and the differentiable
+// mode's (K+1)/2 n^3 ball-pair buffer) are reallocated.
+int set_radius_batch(sf21_hii_ctx* c, int K) {
+  if (K > (int)c->radii.size()) K = (int)c->radii.size();
+  if (K < 1) return SF21_ERR_ARG;
+  if (K == c->K) return SF21_OK;
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return SF21_ERR_HIP;
+  drop_graphs(c);
+  dev_free(c->ubuf);
+  int rc = dev_alloc(&c->ubuf, (size_t)c->n3 * K);
+  if (rc) {
+    // keep the context usable with its previous batch
+    if (dev_alloc(&c->ubuf, (size_t)c->n3 * c->K)) return SF21_ERR_ALLOC;
+    return rc;
+  }
+  c->K = K;
+  if (c->df_v) {
+    dev_free(c->df_v);
+    if ((rc = dev_alloc(&c->df_v, (size_t)(c->K + 1) / 2 * c->n3))) {
+      c->df_ready = false;
+      return rc;
+    }
+  }
+  return SF21_OK;
+}
+
+void apply_choice(sf21_hii_ctx* c, const TuneChoice& t) {
+  c->tile_filter = t.tile;
+  c->filter_w = t.w;
+  c->cols_tile = t.ct;
+  drop_graphs(c);
 }
 
 }  // namespace
@@ -638,9 +731,20 @@ int sf21_hii_create(const sf21_hii_config* cfg, sf21_hii_ctx** out) {
   while ((long)c->h * c->h > wmax) c->h--;
   if (c->h > c->n / 2) c->h = c->n / 2;
   c->K = cfg->radius_batch > 0 ? cfg->radius_batch : 16;
+  bool tuned = false;
+  TuneChoice tc;
+  if (cfg->path == SF21_PATH_FUSED && cfg->radius_batch <= 0 && cfg->filter_tile == 0 &&
+      cfg->filter_w == 0)
+    tuned = tune_lookup(tune_path(nullptr), tune_key(c), &tc);
+  if (tuned) c->K = tc.K;
   if (cfg->filter_tile == 4 || cfg->filter_tile == 8 || cfg->filter_tile == 16)
     c->tile_filter = cfg->filter_tile;
   c->filter_w = cfg->filter_w == 2 ? 0 : 1;
+  if (tuned) {
+    c->tile_filter = tc.tile;
+    c->filter_w = tc.w;
+    c->cols_tile = tc.ct;
+  }
   if (c->K > (int)c->radii.size()) c->K = (int)c->radii.size();
 
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -859,6 +963,108 @@ int sf21_hii_run_batch_dev(sf21_hii_ctx* c, int ns, const float* const* a, const
   return check_hip();
 }
 
+int sf21_hii_run_multi_dev(int nbox, sf21_hii_ctx* const* ctx, const float* const* a,
+                           const float* const* b, const float* const* cc, const double* z,
+                           float* const* xhii, double* neutral) {
+  if (nbox < 1 || !ctx || !a || !b || !z || !xhii) return SF21_ERR_ARG;
+  sf21_hii_ctx* c0 = ctx[0];
+  // per-context neutral slots, in box order
+  std::vector<sf21_hii_ctx*> uniq;
+  std::vector<int> slot(nbox);
+  for (int i = 0; i < nbox; ++i) {
+    sf21_hii_ctx* c = ctx[i];
+    if (!c || !a[i] || !b[i] || !xhii[i] || c->cfg.path != SF21_PATH_FUSED ||
+        c->cfg.device != c0->cfg.device)
+      return SF21_ERR_ARG;
+    int used = 0;
+    for (int k = 0; k < i; ++k) used += ctx[k] == c;
+    if (used >= c->neutral_cap) return SF21_ERR_ARG;
+    slot[i] = used;
+    if (std::find(uniq.begin(), uniq.end(), c) == uniq.end()) uniq.push_back(c);
+  }
+  if (hipSetDevice(c0->cfg.device) != hipSuccess) return SF21_ERR_HIP;
+  (void)hipGetLastError();
+  hipStream_t s = c0->stream;
+  // every context launches on the first one's stream for this call
+  std::vector<hipStream_t> saved;
+  for (auto* c : uniq) {
+    saved.push_back(c->stream);
+    c->stream = s;
+  }
+  bool prof = false;
+  for (auto* c : uniq) prof = prof || c->prof;
+  int rc = SF21_OK;
+  auto enqueue_all = [&] {
+    for (int i = 0; i < nbox; ++i)
+      enqueue_fused(ctx[i], a[i], b[i], cc ? cc[i] : nullptr, z[i], xhii[i],
+                    ctx[i]->neutral_dev + slot[i], -1, nullptr);
+  };
+  if (prof) {
+    enqueue_all();
+  } else {
+    // one hipGraph for the whole set, cached on the first context by
+    // (contexts, pointers, redshifts)
+    std::vector<const void*> key;
+    std::vector<double> zs(z, z + nbox);
+    for (int i = 0; i < nbox; ++i) {
+      key.push_back(ctx[i]);
+      key.push_back(a[i]);
+      key.push_back(b[i]);
+      key.push_back(cc ? cc[i] : nullptr);
+      key.push_back(xhii[i]);
+    }
+    GraphEntry* ge = nullptr;
+    for (auto& g : c0->graphs)
+      if (g.ptrs == key && g.zs == zs) ge = &g;
+    if (!ge) {
+      hipGraph_t graph = nullptr;
+      if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        rc = SF21_ERR_HIP;
+      } else {
+        for (auto* c : uniq) c->capturing = true;
+        enqueue_all();
+        for (auto* c : uniq) c->capturing = false;
+        if (hipStreamEndCapture(s, &graph) != hipSuccess) rc = SF21_ERR_HIP;
+      }
+      if (!rc) {
+        GraphEntry g;
+        g.ptrs = key;
+        g.zs = zs;
+        hipError_t e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) {
+          rc = SF21_ERR_HIP;
+        } else {
+          if (c0->graphs.size() >= 8) {
+            (void)hipGraphExecDestroy(c0->graphs.front().exec);
+            c0->graphs.erase(c0->graphs.begin());
+          }
+          c0->graphs.push_back(g);
+          ge = &c0->graphs.back();
+        }
+      }
+    }
+    if (!rc && hipGraphLaunch(ge->exec, s) != hipSuccess) rc = SF21_ERR_HIP;
+  }
+  if (!rc) {
+    for (auto* c : uniq) {
+      int used = 0;
+      for (int i = 0; i < nbox; ++i) used += ctx[i] == c;
+      if (hipMemcpyAsync(c->neutral_host, c->neutral_dev, sizeof(double) * used,
+                         hipMemcpyDeviceToHost, s) != hipSuccess)
+        rc = SF21_ERR_HIP;
+    }
+    if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = SF21_ERR_HIP;
+  }
+  for (size_t k = 0; k < uniq.size(); ++k) uniq[k]->stream = saved[k];
+  if (rc) return rc;
+  if (neutral)
+    for (int i = 0; i < nbox; ++i) neutral[i] = ctx[i]->neutral_host[slot[i]];
+  for (auto* c : uniq)
+    if (c->prof) collect(c);
+  return check_hip();
+}
+
 int sf21_hii_weight_map(sf21_hii_ctx* c, int* host_out) {
   if (!c || !host_out) return SF21_ERR_ARG;
   if (hipSetDevice(c->cfg.device) != hipSuccess) return SF21_ERR_HIP;
@@ -878,6 +1084,78 @@ int sf21_hii_filtered_dev(sf21_hii_ctx* c, const float* a, const float* b, const
   if (hipStreamSynchronize(c->stream) != hipSuccess) return SF21_ERR_HIP;
   if (c->prof) collect(c);
   return check_hip();
+}
+
+int sf21_hii_autotune(sf21_hii_ctx* c, const float* a, const float* b, const float* cc, double z,
+                      const char* cache_path, double* best_ms) {
+  if (!c || !a || !b || c->cfg.path != SF21_PATH_FUSED) return SF21_ERR_ARG;
+  if (hipSetDevice(c->cfg.device) != hipSuccess) return SF21_ERR_HIP;
+  (void)hipGetLastError();
+  float* xout = nullptr;
+  int rc = dev_alloc(&xout, (size_t)c->n3);
+  if (rc) return rc;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    dev_free(xout);
+    return SF21_ERR_HIP;
+  }
+  const bool prof = c->prof;
+  c->prof = false;
+  TuneChoice best;
+  double best_t = 1e300;
+  const int nr = (int)c->radii.size();
+  std::vector<int> ks;
+  for (int k : {8, 12, 16})
+    if (std::find(ks.begin(), ks.end(), std::min(k, nr)) == ks.end()) ks.push_back(std::min(k, nr));
+  for (int K : ks) {
+    if (set_radius_batch(c, K)) continue;  // does not fit: skip
+    for (int w : {1, 0})
+      for (int tile : {16, 8})
+        for (int ct : {16, 8}) {
+          TuneChoice t;
+          t.tile = tile;
+          t.K = c->K;
+          t.w = w;
+          t.ct = ct;
+          apply_choice(c, t);
+          enqueue_fused(c, a, b, cc, z, xout, c->neutral_dev, -1, nullptr);  // warm
+          (void)hipEventRecord(e0, c->stream);
+          for (int r = 0; r < 2; ++r)
+            enqueue_fused(c, a, b, cc, z, xout, c->neutral_dev, -1, nullptr);
+          (void)hipEventRecord(e1, c->stream);
+          float ms = 0.f;
+          if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+            rc = SF21_ERR_HIP;
+            break;
+          }
+          if (ms / 2 < best_t) {
+            best_t = ms / 2;
+            best = t;
+          }
+        }
+    if (rc) break;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  dev_free(xout);
+  c->prof = prof;
+  if (rc) return rc;
+  if (best_t >= 1e300) return SF21_ERR_ALLOC;
+  if ((rc = set_radius_batch(c, best.K))) return rc;
+  apply_choice(c, best);
+  tune_store(tune_path(cache_path), tune_key(c), best, best_t);
+  if (best_ms) *best_ms = best_t;
+  return check_hip();
+}
+
+int sf21_hii_tuning(const sf21_hii_ctx* c, int* filter_tile, int* radius_batch, int* filter_w,
+                    int* cols_tile) {
+  if (!c) return SF21_ERR_ARG;
+  if (filter_tile) *filter_tile = c->tile_filter;
+  if (radius_batch) *radius_batch = c->K;
+  if (filter_w) *filter_w = c->filter_w ? 1 : 2;
+  if (cols_tile) *cols_tile = c->cols_tile;
+  return SF21_OK;
 }
 
 int sf21_hii_radius_batch(const sf21_hii_ctx* c) { return c ? c->K : SF21_ERR_ARG; }

@@ -1,0 +1,54 @@
+"""Autotuner of the fused path (sf21_hii_autotune): the winner is one of the
+candidates, it is recorded in the cache file, a new context created with
+automatic settings picks it up from $SF21_TUNE_CACHE, and the tuned context
+still matches the oracle."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle_bind as orc
+import simfast21_amd as sf
+from helpers import direct_case
+
+pytestmark = pytest.mark.gpu
+
+
+def test_autotune_records_and_reloads(torch_cuda, tmp_path):
+    torch = torch_cuda
+    case = direct_case(96, zeta=1.0, seed=8100)
+    n, side, radii = case["n"], case["side"], case["radii"]
+    a = torch.from_numpy(case["rec"].ravel()).cuda()
+    b = torch.from_numpy(case["ion"].ravel()).cuda()
+    cache = tmp_path / "tune.txt"
+    ctx = sf.HiiContext(n, side, radii, sf.INPUT_DIRECT, sf.PATH_FUSED)
+    ms = ctx.autotune(a, b, cache_path=cache)
+    t = ctx.tuning()
+    print(f"\nautotune {n}^3, {len(radii)} radii: {ms:.3f} ms/slice with {t}", flush=True)
+    assert ms > 0
+    assert t["filter_tile"] in (8, 16) and t["cols_tile"] in (8, 16) and t["filter_w"] in (1, 2)
+    assert t["radius_batch"] in (8, 12, 16, len(radii))
+    lines = cache.read_text().strip().splitlines()
+    assert len(lines) == 1 and f"n={n} radii={len(radii)}" in lines[0]
+    x, neu = ctx.run_slice(a, b)
+    xo, neu_o, _ = orc.hii_slice(n, side, radii, case["rec"], case["ion"], precision=64, dilation=1)
+    D = int(np.count_nonzero(np.abs(x.cpu().numpy() - xo) > 1e-4))
+    assert D / n ** 3 < 1e-4 and abs(neu - neu_o) <= max(1e-5, D / n ** 3 + 1e-12)
+    ctx.close()
+    old = os.environ.get("SF21_TUNE_CACHE")
+    os.environ["SF21_TUNE_CACHE"] = str(cache)
+    try:
+        ctx2 = sf.HiiContext(n, side, radii, sf.INPUT_DIRECT, sf.PATH_FUSED)
+        assert ctx2.tuning() == t
+        x2, neu2 = ctx2.run_slice(a, b)
+        assert np.array_equal(x2.cpu().numpy(), x.cpu().numpy()) and neu2 == neu
+        ctx2.close()
+        # explicit settings are not overridden by the cache
+        ctx3 = sf.HiiContext(n, side, radii, sf.INPUT_DIRECT, sf.PATH_FUSED, radius_batch=4)
+        assert ctx3.tuning()["radius_batch"] == 4
+        ctx3.close()
+    finally:
+        if old is None:
+            del os.environ["SF21_TUNE_CACHE"]
+        else:
+            os.environ["SF21_TUNE_CACHE"] = old

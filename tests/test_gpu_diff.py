@@ -5,8 +5,9 @@ down by 0.9 to the cell size), zeta = 50 (<x> ~ 0.5):
   * the relaxed adjoint against autograd of the FP64 torch reference
     (tests/diff_reference.py) at tau in {0.05, 0.02, 0.01}: gradient in g
     (relative L2 error), gradient in zeta, relaxed <x~> and P_b;
-  * central differences of the GPU's own relaxed loss along zeta (relative
-    step 1e-3: zeta is a scale);
+  * central differences along zeta: the FP64 reference's (step 1e-3) against
+    the GPU adjoint, the GPU's own relaxed loss (step 1e-2, above its FP32
+    noise floor) against the FP64 one;
   * the exact forward: <x> and P_b against the FP64 oracle's bubble finder
     on the same fields (FP32-undecidable flips allowed through the mean);
   * tau -> 0: the relaxed <x~> tends to the exact <x>.
@@ -82,12 +83,13 @@ def test_adjoint_matches_fp64_reference(setup, tau):
 
 
 def test_zeta_gradient_vs_central_differences(setup):
-    """Central differences along zeta, step 1e-3 relative (zeta is a scale).
-    Loss: <x~> plus the mean relative change of P_b.  The FP64 reference's
-    central difference pins the GPU adjoint; the GPU's own central difference
-    is compared with it at its FP32 floor (the loss carries ~1e-5 of
-    evaluation noise, decorrelated between zeta +- h, i.e. ~1e-4 on a
-    derivative of ~1e-3)."""
+    """Central differences along zeta (zeta is a scale, so steps are
+    relative).  Loss: <x~> plus the mean relative change of P_b.
+      * FP64 reference, step 1e-3: pins the GPU adjoint (within 1e-2);
+      * the GPU's own relaxed loss carries ~1e-5 of FP32 evaluation noise,
+        decorrelated between zeta +- h: at step 1e-3 that is ~20% of this
+        derivative (~1e-3), so its central difference is taken at step 1e-2
+        and compared with the FP64 one at the same step (within 5e-2)."""
     s = setup
     tau = 0.05
     ctx = s["ctx"]
@@ -104,14 +106,18 @@ def test_zeta_gradient_vs_central_differences(setup):
             X, P, _ = dr.relaxed(s["ref"], gt, z, tau, NB)
         return float(X) + float(np.dot(cp, P.numpy()))
 
+    def cd(f, rel):
+        h = rel * ZETA
+        return (f(ZETA + h) - f(ZETA - h)) / (2 * h)
+
     _, gz, _, _ = ctx.diff_backward(s["gd"], s["ed"], ZETA, tau, 1.0, cp, NB)
-    h = 1e-3 * ZETA
-    fd_gpu = (loss_gpu(ZETA + h) - loss_gpu(ZETA - h)) / (2 * h)
-    fd_ref = (loss_ref(ZETA + h) - loss_ref(ZETA - h)) / (2 * h)
-    print(f"\nzeta: adjoint {gz:.6e}  central difference fp64 {fd_ref:.6e}  gpu {fd_gpu:.6e}",
-          flush=True)
-    assert abs(fd_ref - gz) <= 1e-2 * abs(gz)
-    assert abs(fd_gpu - fd_ref) <= 1.5e-1 * abs(fd_ref)
+    fd_ref3 = cd(loss_ref, 1e-3)
+    fd_ref2 = cd(loss_ref, 1e-2)
+    fd_gpu2 = cd(loss_gpu, 1e-2)
+    print(f"\nzeta: adjoint {gz:.6e}  central difference fp64 step 1e-3 {fd_ref3:.6e}, "
+          f"step 1e-2 fp64 {fd_ref2:.6e} gpu {fd_gpu2:.6e}", flush=True)
+    assert abs(fd_ref3 - gz) <= 1e-2 * abs(gz)
+    assert abs(fd_gpu2 - fd_ref2) <= 5e-2 * abs(fd_ref2)
 
 
 def test_exact_forward_matches_oracle(setup):
