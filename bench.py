@@ -205,6 +205,22 @@ def bench_diff(torch, sf, n, side, radii, n_src, taus=(0.05, 0.02, 0.01), reps=3
         bwd[str(tau)] = round((time.perf_counter() - t0) * 1e3 / reps, 2)
         print(f"[bench] diff tau={tau}: backward {bwd[str(tau)]:.1f} ms, <x~> {mxt:.4f}, "
               f"dL/dzeta {gz:.4e}", file=sys.stderr, flush=True)
+    # field-storage mode: the second sweep reads the first sweep's pairs
+    bwd_stored = None
+    try:
+        ctx.diff_store_fields(True)
+        bwd_stored = {}
+        for tau in taus:
+            ctx.diff_backward(g, emis, zeta, tau, 1.0, cp, 20, grad_out=grad)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                ctx.diff_backward(g, emis, zeta, tau, 1.0, cp, 20, grad_out=grad)
+            torch.cuda.synchronize()
+            bwd_stored[str(tau)] = round((time.perf_counter() - t0) * 1e3 / reps, 2)
+        ctx.diff_store_fields(False)
+    except sf.Sf21Error as e:  # does not fit
+        bwd_stored = {"error": str(e)}
     # per-kernel split of one backward
     ctx.reset_stats()
     ctx.profile(True)
@@ -221,6 +237,9 @@ def bench_diff(torch, sf, n, side, radii, n_src, taus=(0.05, 0.02, 0.01), reps=3
         "backward_ms": bwd,
         "grad_over_forward": {k: round(v / fwd_ms, 3) for k, v in bwd.items()},
         "grad_over_forward_max": round(worst / fwd_ms, 3),
+        "backward_ms_fields_stored": bwd_stored,
+        "grad_over_forward_fields_stored": ({k: round(v / fwd_ms, 3) for k, v in bwd_stored.items()}
+                                            if bwd_stored and "error" not in bwd_stored else None),
         "backward_kernels_ms": {k: [c, round(t, 3)] for k, (c, t) in kst.items()},
     }
 

@@ -148,8 +148,14 @@ int sf21_hii_power_spectrum_dev(sf21_hii_ctx *ctx, const float *field, int nbins
  * the union of balls becomes x~ = q + (1 - q)(1 - exp(-sum_R B_R * phi_R)),
  * B_R the exact ball, q the quick fill) with respect to g (grad_g, device
  * n^3) and zeta (*grad_zeta), plus the relaxed <x~> and P_b (host,
- * nullable).  Per-radius fields are recomputed in the backward, never
- * stored; tau -> 0 gives back the exact forward.  Sums in FP64. */
+ * nullable).  Per-radius fields are recomputed in the backward unless
+ * sf21_hii_diff_store_fields is on; tau -> 0 gives back the exact forward.
+ * Sums in FP64. */
+/* Field-storage mode of the backward (default off): the first sweep keeps
+ * every radius' filtered pair (n_R n^3 complex floats, 47 GB at 512^3 with 47
+ * radii) and the second sweep reads them instead of recomputing them.
+ * SF21_ERR_ALLOC if they do not fit; the gradient is identical. */
+int sf21_hii_diff_store_fields(sf21_hii_ctx *ctx, int enable);
 int sf21_hii_diff_forward_dev(sf21_hii_ctx *ctx, const float *g, const float *emis, double zeta,
                               int nbins, float *x_out, double *mean_x, double *pk);
 int sf21_hii_diff_backward_dev(sf21_hii_ctx *ctx, const float *g, const float *emis,
@@ -169,10 +175,13 @@ int sf21_hii_filtered_dev(sf21_hii_ctx *ctx, const float *a, const float *b, con
 
 /* Autotuner (fused path; no counterpart in the reference, whose FFTW plans
  * are tuned by FFTW_MEASURE at get_HIIbubbles.c:110-183).  Times the slice
- * (a, b, c, z; device pointers) for every candidate -- filter tile 16/8,
- * radius batch 8/12/16 (capped by the ladder; skipped when it does not fit),
- * top-hat weights in the kernel or from the FP64 table, cols tile 16/8 --
- * keeps the fastest in the context and appends it to the cache file
+ * (a, b, c, z; device pointers) by a coordinate search over: radius batch
+ * 16/12/8 (capped by the ladder; skipped when it does not fit), top-hat
+ * weights in the kernel or from the FP64 table, filter tile 16/8, the
+ * per-radius column passes (8 values per thread with 8 or 16 columns per
+ * block, or 16 values per thread with 16 or 32 columns; the latter for
+ * n = 128, 256, 512) and the centre-test pass (8 or 16 values per thread).
+ * Keeps the fastest in the context and appends it to the cache file
  * (cache_path, else $SF21_TUNE_CACHE, else not recorded) keyed by
  * (device arch, n, number of radii).  A context created with radius_batch,
  * filter_tile and filter_w all 0 takes the cached choice for its key from
@@ -180,9 +189,10 @@ int sf21_hii_filtered_dev(sf21_hii_ctx *ctx, const float *a, const float *b, con
 int sf21_hii_autotune(sf21_hii_ctx *ctx, const float *a, const float *b, const float *c,
                       double z, const char *cache_path, double *best_ms);
 /* Current choices: filter tile, radius batch, top-hat weights (1 in the
- * kernel, 2 table), cols tile; any pointer may be NULL. */
+ * kernel, 2 table), column-pass tile, 16 values per thread in the column
+ * passes (0/1) and in the centre-test pass (0/1); any pointer may be NULL. */
 int sf21_hii_tuning(const sf21_hii_ctx *ctx, int *filter_tile, int *radius_batch, int *filter_w,
-                    int *cols_tile);
+                    int *cols_tile, int *cols_wide, int *rows_wide);
 
 /* Effective radii per fused filter pass (the automatic choice depends on
  * free device memory); negative on error. */

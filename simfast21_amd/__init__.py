@@ -115,11 +115,12 @@ def _lib():
     L.sf21_hii_profile.argtypes = [vp, i]
     L.sf21_hii_radius_batch.argtypes = [vp]
     L.sf21_hii_autotune.argtypes = [vp, vp, vp, vp, d, ctypes.c_char_p, dp]
-    L.sf21_hii_tuning.argtypes = [vp, ip, ip, ip, ip]
+    L.sf21_hii_tuning.argtypes = [vp, ip, ip, ip, ip, ip, ip]
     L.sf21_hii_kernel_stats.argtypes = [vp, i, ctypes.c_char_p, i, ctypes.POINTER(ctypes.c_long), dp]
     L.sf21_hii_reset_stats.argtypes = [vp]
     L.sf21_hii_power_spectrum_dev.argtypes = [vp, vp, i, vp, vp, vp]
     L.sf21_hii_halo_emissivity_dev.argtypes = [vp, vp, ctypes.c_long, d, vp]
+    L.sf21_hii_diff_store_fields.argtypes = [vp, i]
     L.sf21_hii_diff_forward_dev.argtypes = [vp, vp, vp, d, i, vp, dp, vp]
     L.sf21_hii_diff_backward_dev.argtypes = [vp, vp, vp, d, d, i, d, vp, vp, dp, dp, vp]
     L.sf21_hii_run_slice_halo.argtypes = [vp, vp, vp, ctypes.c_long, d, vp, dp]
@@ -278,6 +279,11 @@ class HiiContext:
                "sf21_hii_power_spectrum_dev")
         return pk, km, ct
 
+    def diff_store_fields(self, enable: bool = True):
+        """Keep the backward's per-radius filtered pairs between its two sweeps
+        (n_R n^3 complex floats) instead of recomputing them."""
+        _check(_lib().sf21_hii_diff_store_fields(self._h, int(enable)), "sf21_hii_diff_store_fields")
+
     def diff_forward(self, g, emis, zeta: float, nbins: int = 20, x_out=None):
         """Exact forward of the differentiable model on device boxes g, emis:
         (x, mean_x, pk).  See sf21_hii_diff_forward_dev."""
@@ -336,10 +342,10 @@ class HiiContext:
         return ms.value
 
     def tuning(self) -> dict:
-        v = [ctypes.c_int() for _ in range(4)]
+        v = [ctypes.c_int() for _ in range(6)]
         _check(_lib().sf21_hii_tuning(self._h, *[ctypes.byref(x) for x in v]), "sf21_hii_tuning")
         return {"filter_tile": v[0].value, "radius_batch": v[1].value, "filter_w": v[2].value,
-                "cols_tile": v[3].value}
+                "cols_tile": v[3].value, "cols_wide": v[4].value, "rows_wide": v[5].value}
 
     @property
     def radius_batch(self) -> int:

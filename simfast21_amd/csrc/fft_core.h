@@ -219,6 +219,15 @@ __device__ __forceinline__ void line_fft_impl(float2 (&v)[E], float* sre, float*
   stages<L, E, SIGN, 1, Rs...>(v, sre, sim, t, tw);
 }
 
+// Line transform with E values per thread (every radix of the size's plan
+// must divide E): E = 16 halves the threads per line and doubles the work
+// between the exchanges' barriers.
+template <int L, int E, int SIGN>
+__device__ __forceinline__ void line_fft_e(float2 (&v)[E], float* sre, float* sim, int t,
+                                           const float2* __restrict__ tw) {
+  line_fft_impl<L, E, SIGN>(v, sre, sim, t, tw, typename Radices<L>::type{});
+}
+
 // Full line transform.  Every thread of the block must call it (it contains
 // block barriers), including threads whose line is out of range.
 template <int L, int SIGN>

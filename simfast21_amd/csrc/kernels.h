@@ -26,14 +26,17 @@ void launch_balance(long n3, const float* a, const float* b, const float* c, Cel
                     double* part, float* scale, hipStream_t s);
 void launch_fwd_rows(int n, const float* a, const float* b, const float* c, CellConsts k,
                      const float* bal, float2* y, const float2* tw, hipStream_t s);
+// tile: columns per block (8, 16; 16 or 32 when wide); wide: 16 values per
+// thread instead of the size's default (sizes 128, 256, 512)
 void launch_cols(int n, int width, int axis, int sign, const float2* in, float2* out,
-                 int nbatch, long bstride, const float2* tw, hipStream_t s, int tile = 16);
+                 int nbatch, long bstride, const float2* tw, hipStream_t s, int tile = 16,
+                 bool wide = false);
 void launch_filter_i(int n, const float2* spec, float2* u, int nrad, long ustride,
                      const float* wlut, long m_lut, const float2* kappa, const float2* tw,
                      hipStream_t s, int tile, int wmode);
 void launch_inv_rows_crit(int n, const float2* v, int nrad, long vstride, const int* d2max,
                           const float* bal, int* wmap, float2* dbg, hipStream_t s,
-                          const float2* tw);
+                          const float2* tw, bool wide = false);
 void launch_fill_i32(int* p, int value, long count, hipStream_t s);
 void launch_dt_rows(const DtArgs& a, const int* in, int* out, hipStream_t s);
 void launch_dt_cols(const DtArgs& a, const int* in, int* out, hipStream_t s);
@@ -91,11 +94,13 @@ void launch_t_field(long n3, const float* x, const float* delta1, float* t, doub
                     double* st, hipStream_t s);
 void launch_ball_box(int n, int d2, float2* out, hipStream_t s);
 void launch_ball_fold(int n, const float2* spec, float* table, hipStream_t s);
+// pairs (nullable): per-radius filtered pairs, stored by rows_phi and read
+// by rows_seed instead of a recomputation (field-storage mode)
 void launch_rows_phi(int n, float2* u, int nrad, long ustride, const float* bal, const double* st,
-                     float inv_tau, const float2* tw, hipStream_t s);
+                     float inv_tau, const float2* tw, float2* pairs, long pstride, hipStream_t s);
 void launch_rows_seed(int n, float2* u, const float2* v, int nrad, long ustride, long vstride,
                       const float* bal, const double* st, float inv_tau, double* rho_acc,
-                      const float2* tw, hipStream_t s);
+                      const float2* tw, const float2* pairs, long pstride, hipStream_t s);
 void launch_ball_i(int n, bool acc, const float2* in, long istride, float2* out, long ostride,
                    int nrad, const float* bt, const float2* tw, hipStream_t s);
 void launch_filter_acc_i(int n, const float2* in, long istride, int nrad, const float* wlut,

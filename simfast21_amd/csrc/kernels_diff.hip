@@ -187,7 +187,8 @@ template <int L>
 __global__ __launch_bounds__(256) void k_rows_phi(float2* u, int nrad, long ustride,
                                                   const float* __restrict__ bal,
                                                   const double* __restrict__ st, float inv_tau,
-                                                  const float2* __restrict__ tw) {
+                                                  const float2* __restrict__ tw,
+                                                  float2* __restrict__ pairs, long pstride) {
   constexpr int E = Plan<L>::E, TPL = L / E, LPB = 256 / TPL;
   __shared__ float lds[LPB * 2 * LdsLine<L>::kFloats];
   const int lw = threadIdx.x / TPL, t = threadIdx.x % TPL;
@@ -207,6 +208,12 @@ __global__ __launch_bounds__(256) void k_rows_phi(float2* u, int nrad, long ustr
 #pragma unroll
     for (int m = 0; m < E; ++m) v[m] = src[tl + TPL * m];
     line_fft<L, +1>(v, sre, sim, tl, tw);
+    if (pairs) {
+      // stored for the second sweep (field-storage mode)
+      float2* dst = pairs + (long)r * pstride + row * L;
+#pragma unroll
+      for (int m = 0; m < E; ++m) dst[tl + TPL * m] = v[m];
+    }
     float ph[E];
 #pragma unroll
     for (int m = 0; m < E; ++m) {
@@ -314,13 +321,14 @@ __global__ __launch_bounds__(TILE*(L / Plan<L>::E)) void k_ball_i(
 // rho_acc[block] (accumulated over the batches).  In unnormalised units the
 // factors N of ds/d(.) and of phibar cancel.
 // ---------------------------------------------------------------------------
-template <int L>
+template <int L, bool STORED>
 __global__ __launch_bounds__(256) void k_rows_seed(float2* u, const float2* v, int nrad,
                                                    long ustride, long vstride,
                                                    const float* __restrict__ bal,
                                                    const double* __restrict__ st, float inv_tau,
                                                    double* __restrict__ rho_acc,
-                                                   const float2* __restrict__ tw) {
+                                                   const float2* __restrict__ tw,
+                                                   const float2* __restrict__ pairs, long pstride) {
   constexpr int E = Plan<L>::E, TPL = L / E, LPB = 256 / TPL;
   __shared__ float lds[LPB * 2 * LdsLine<L>::kFloats];
   __shared__ double r1[256], r2[256];
@@ -348,9 +356,16 @@ __global__ __launch_bounds__(256) void k_rows_seed(float2* u, const float2* v, i
       const int tl = opaque_lane(t);
       float2 y[E];
       float2* src = u + (long)(2 * pr + hh) * ustride + row * L;
+      if constexpr (STORED) {
+        // the pair as the first sweep left it (already inverse-transformed)
+        const float2* ps = pairs + (long)(2 * pr + hh) * pstride + row * L;
 #pragma unroll
-      for (int m = 0; m < E; ++m) y[m] = src[tl + TPL * m];
-      line_fft<L, +1>(y, sre, sim, tl, tw);
+        for (int m = 0; m < E; ++m) y[m] = ps[tl + TPL * m];
+      } else {
+#pragma unroll
+        for (int m = 0; m < E; ++m) y[m] = src[tl + TPL * m];
+        line_fft<L, +1>(y, sre, sim, tl, tw);
+      }
 #pragma unroll
       for (int m = 0; m < E; ++m) {
         const float ru = y[m].x, iu = y[m].y * inv_s;
@@ -626,18 +641,23 @@ void launch_ball_fold(int n, const float2* spec, float* table, hipStream_t s) {
 
 template <int L>
 static void rows_phi_t(int n, float2* u, int nrad, long ustride, const float* bal, const double* st,
-                       float inv_tau, const float2* tw, hipStream_t s) {
+                       float inv_tau, const float2* tw, float2* pairs, long pstride, hipStream_t s) {
   constexpr int LPB = 256 / (L / Plan<L>::E);
   hipLaunchKernelGGL(k_rows_phi<L>, dim3((unsigned)((long)n * n / LPB)), dim3(256), 0, s, u, nrad,
-                     ustride, bal, st, inv_tau, tw);
+                     ustride, bal, st, inv_tau, tw, pairs, pstride);
 }
 template <int L>
 static void rows_seed_t(int n, float2* u, const float2* v, int nrad, long ustride, long vstride,
                         const float* bal, const double* st, float inv_tau, double* rho_acc,
-                        const float2* tw, hipStream_t s) {
+                        const float2* tw, const float2* pairs, long pstride, hipStream_t s) {
   constexpr int LPB = 256 / (L / Plan<L>::E);
-  hipLaunchKernelGGL(k_rows_seed<L>, dim3((unsigned)((long)n * n / LPB)), dim3(256), 0, s, u, v,
-                     nrad, ustride, vstride, bal, st, inv_tau, rho_acc, tw);
+  const dim3 g((unsigned)((long)n * n / LPB));
+  if (pairs)
+    hipLaunchKernelGGL((k_rows_seed<L, true>), g, dim3(256), 0, s, u, v, nrad, ustride, vstride, bal,
+                       st, inv_tau, rho_acc, tw, pairs, pstride);
+  else
+    hipLaunchKernelGGL((k_rows_seed<L, false>), g, dim3(256), 0, s, u, v, nrad, ustride, vstride,
+                       bal, st, inv_tau, rho_acc, tw, pairs, pstride);
 }
 template <int L, bool ACC>
 static void ball_i_t(const float2* in, long istride, float2* out, long ostride, int nrad,
@@ -680,18 +700,18 @@ int rows_blocks(int n) {
   return 0;
 }
 void launch_rows_phi(int n, float2* u, int nrad, long ustride, const float* bal, const double* st,
-                     float inv_tau, const float2* tw, hipStream_t s) {
+                     float inv_tau, const float2* tw, float2* pairs, long pstride, hipStream_t s) {
   switch (n) {
-#define C(L) case L: rows_phi_t<L>(n, u, nrad, ustride, bal, st, inv_tau, tw, s); break;
+#define C(L) case L: rows_phi_t<L>(n, u, nrad, ustride, bal, st, inv_tau, tw, pairs, pstride, s); break;
     SF21_DIFF_SIZES(C)
 #undef C
   }
 }
 void launch_rows_seed(int n, float2* u, const float2* v, int nrad, long ustride, long vstride,
                       const float* bal, const double* st, float inv_tau, double* rho_acc,
-                      const float2* tw, hipStream_t s) {
+                      const float2* tw, const float2* pairs, long pstride, hipStream_t s) {
   switch (n) {
-#define C(L) case L: rows_seed_t<L>(n, u, v, nrad, ustride, vstride, bal, st, inv_tau, rho_acc, tw, s); break;
+#define C(L) case L: rows_seed_t<L>(n, u, v, nrad, ustride, vstride, bal, st, inv_tau, rho_acc, tw, pairs, pstride, s); break;
     SF21_DIFF_SIZES(C)
 #undef C
   }

@@ -171,12 +171,12 @@ __global__ __launch_bounds__(256) void k_fwd_rows(const float* __restrict__ a,
 // L x L x width array; TILE adjacent columns per block (128 B runs).
 // blockIdx.y selects one of several arrays spaced bstride apart.
 // ------------------------------------------------------------------------
-template <int L, int SIGN, int TILE>
-__global__ __launch_bounds__(TILE*(L / Plan<L>::E)) void k_cols(const float2* in, float2* out,
-                                                                int width, int axis,
-                                                                long bstride,
-                                                                const float2* __restrict__ tw) {
-  constexpr int E = Plan<L>::E, TPL = L / E;
+template <int L, int SIGN, int TILE, int E = Plan<L>::E>
+__global__ __launch_bounds__(TILE*(L / E)) void k_cols(const float2* in, float2* out,
+                                                       int width, int axis,
+                                                       long bstride,
+                                                       const float2* __restrict__ tw) {
+  constexpr int TPL = L / E;
   __shared__ float lds[TILE * 2 * LdsLine<L>::kFloats];
   const int c = threadIdx.x % TILE, t = threadIdx.x / TILE;
   const int ntile = (width + TILE - 1) / TILE;
@@ -202,7 +202,7 @@ __global__ __launch_bounds__(TILE*(L / Plan<L>::E)) void k_cols(const float2* in
 #pragma unroll
     for (int m = 0; m < E; ++m) v[m] = make_float2(0.f, 0.f);
   }
-  line_fft<L, SIGN>(v, sre, sim, t, tw);
+  line_fft_e<L, E, SIGN>(v, sre, sim, t, tw);
   if (ok) {
 #pragma unroll
     for (int m = 0; m < E; ++m) out[base + (long)(t + TPL * m) * stride] = v[m];
@@ -269,7 +269,7 @@ __global__ __launch_bounds__(TILE*(L / Plan<L>::E)) void k_filter_i(
 // the largest radius -- no atomics and no reads of the map.  dbg (nullable)
 // receives the unscaled filtered pairs.
 // ------------------------------------------------------------------------
-template <int L>
+template <int L, int E = Plan<L>::E>
 __global__ __launch_bounds__(256) void k_inv_rows_crit(const float2* __restrict__ vin,
                                                        int nrad, long vstride,
                                                        const int* __restrict__ d2max,
@@ -277,7 +277,7 @@ __global__ __launch_bounds__(256) void k_inv_rows_crit(const float2* __restrict_
                                                        int* __restrict__ wmap,
                                                        float2* __restrict__ dbg,
                                                        const float2* __restrict__ tw) {
-  constexpr int E = Plan<L>::E, TPL = L / E, LPB = 256 / TPL;
+  constexpr int TPL = L / E, LPB = 256 / TPL;
   __shared__ float lds[LPB * 2 * LdsLine<L>::kFloats];
   const int lw = threadIdx.x / TPL, t = threadIdx.x % TPL;
   const long row = (long)blockIdx.x * LPB + lw;
@@ -294,7 +294,7 @@ __global__ __launch_bounds__(256) void k_inv_rows_crit(const float2* __restrict_
     float2 v[E];
 #pragma unroll
     for (int m = 0; m < E; ++m) v[m] = src[tl + TPL * m];
-    line_fft<L, +1>(v, sre, sim, tl, tw);
+    line_fft_e<L, E, +1>(v, sre, sim, tl, tw);
     const int w = d2max[r];
 #pragma unroll
     for (int m = 0; m < E; ++m) {
@@ -579,20 +579,21 @@ void launch_fwd_rows(int n, const float* a, const float* b, const float* c, Cell
 #undef X
 }
 
-template <int L, int TILE>
+template <int L, int TILE, int E = Plan<L>::E>
 static void cols_t(int width, int axis, int sign, const float2* in, float2* out, int nbatch,
                    long bstride, const float2* tw, hipStream_t s) {
-  constexpr int NT = TILE * (L / Plan<L>::E);
+  constexpr int NT = TILE * (L / E);
   dim3 g(L * ((width + TILE - 1) / TILE), nbatch);
   if (sign < 0)
-    hipLaunchKernelGGL((k_cols<L, -1, TILE>), g, dim3(NT), 0, s, in, out, width, axis, bstride, tw);
+    hipLaunchKernelGGL((k_cols<L, -1, TILE, E>), g, dim3(NT), 0, s, in, out, width, axis, bstride, tw);
   else
-    hipLaunchKernelGGL((k_cols<L, +1, TILE>), g, dim3(NT), 0, s, in, out, width, axis, bstride, tw);
+    hipLaunchKernelGGL((k_cols<L, +1, TILE, E>), g, dim3(NT), 0, s, in, out, width, axis, bstride, tw);
 }
 
-// tile: columns per block, 16 (default) or 8
+// tile: columns per block, 16 (default) or 8; wide: 16 values per thread
+// (sizes 128-512; tile 16 or 32)
 void launch_cols(int n, int width, int axis, int sign, const float2* in, float2* out,
-                 int nbatch, long bstride, const float2* tw, hipStream_t s, int tile) {
+                 int nbatch, long bstride, const float2* tw, hipStream_t s, int tile, bool wide) {
 #define X(L)                                                                              \
   if (n == L) {                                                                           \
     if (tile == 8)                                                                        \
@@ -601,7 +602,17 @@ void launch_cols(int n, int width, int axis, int sign, const float2* in, float2*
       cols_t<L, 16>(width, axis, sign, in, out, nbatch, bstride, tw, s);                  \
     return;                                                                               \
   }
+#define XW(L)                                                                             \
+  if (n == L && wide) {                                                                   \
+    if (tile == 32)                                                                       \
+      cols_t<L, 32, 16>(width, axis, sign, in, out, nbatch, bstride, tw, s);              \
+    else                                                                                  \
+      cols_t<L, 16, 16>(width, axis, sign, in, out, nbatch, bstride, tw, s);              \
+    return;                                                                               \
+  }
+  XW(128) XW(256) XW(512)
   SF21_SIZES(X)
+#undef XW
 #undef X
 }
 
@@ -627,7 +638,14 @@ void launch_filter_i(int n, const float2* spec, float2* u, int nrad, long ustrid
 
 void launch_inv_rows_crit(int n, const float2* v, int nrad, long vstride, const int* d2max,
                           const float* bal, int* wmap, float2* dbg, hipStream_t s,
-                          const float2* tw) {
+                          const float2* tw, bool wide) {
+#define XW(L)                                                                             \
+  if (n == L && wide) {                                                                   \
+    constexpr int LPB = 256 / (L / 16);                                                   \
+    hipLaunchKernelGGL((k_inv_rows_crit<L, 16>), dim3((L * L) / LPB), dim3(256), 0, s, v, nrad, \
+                       vstride, d2max, bal, wmap, dbg, tw);                               \
+    return;                                                                               \
+  }
 #define X(L)                                                                              \
   if (n == L) {                                                                           \
     constexpr int LPB = 256 / (L / Plan<L>::E);                                           \
@@ -635,7 +653,9 @@ void launch_inv_rows_crit(int n, const float2* v, int nrad, long vstride, const 
                        d2max, bal, wmap, dbg, tw);                                        \
     return;                                                                               \
   }
+  XW(128) XW(256) XW(512)
   SF21_SIZES(X)
+#undef XW
 #undef X
 }
 

@@ -77,7 +77,9 @@ struct sf21_hii_ctx {
   std::vector<int> d2;
   int n = 0, h = 0, K = 0;
   int tile_filter = 16;
-  int cols_tile = 16;  // columns per block of the per-radius inverse along j
+  int cols_tile = 16;  // columns per block of the per-radius passes along j
+  bool cols_wide = false;  // 16 values per thread in the per-radius passes along j
+  bool rows_wide = false;  // 16 values per thread in the centre-test pass
   int filter_w = 1;  // 1: FP32 on the fly, 0: FP64 table per |k|^2
   float2* d_kappa = nullptr;  // dk R per radius as a float pair (hi, lo)
   long n3 = 0, nh = 0;
@@ -121,6 +123,7 @@ struct sf21_hii_ctx {
   double *df_st = nullptr, *df_part = nullptr, *df_rho = nullptr, *df_gp = nullptr;
   double* df_coef = nullptr;
   double* df_host = nullptr;  // pinned staging
+  float2* df_pairs = nullptr; // field-storage mode: n_R x n^3 filtered pairs
   bool df_ready = false;
   // halo mode
   float* halo_grid = nullptr;   // N_halo^3 deposit grid
@@ -269,9 +272,9 @@ void enqueue_fused(sf21_hii_ctx* c, const float* a, const float* b, const float*
                       c->d_kappa + rb, c->tw, s, c->tile_filter, c->filter_w);
     });
     launch(c, "cols_inv_j",
-           [&] { launch_cols(n, n, 0, +1, c->ubuf, c->ubuf, kk, n3, c->tw, s, c->cols_tile); });
+           [&] { launch_cols(n, n, 0, +1, c->ubuf, c->ubuf, kk, n3, c->tw, s, c->cols_tile, c->cols_wide); });
     launch(c, "inv_rows_crit", [&] {
-      launch_inv_rows_crit(n, c->ubuf, kk, n3, c->d_d2 + rb, c->bal, c->wmap, dbg, s, c->tw);
+      launch_inv_rows_crit(n, c->ubuf, kk, n3, c->d_d2 + rb, c->bal, c->wmap, dbg, s, c->tw, c->rows_wide);
     });
   }
   if (only_radius >= 0) return;
@@ -377,6 +380,7 @@ void free_all(sf21_hii_ctx* c) {
   dev_free(c->df_dbar); dev_free(c->df_dd); dev_free(c->df_v); dev_free(c->df_c);
   dev_free(c->df_g); dev_free(c->df_half); dev_free(c->df_ball); dev_free(c->df_st);
   dev_free(c->df_part); dev_free(c->df_rho); dev_free(c->df_gp); dev_free(c->df_coef);
+  dev_free(c->df_pairs);
   if (c->df_host) (void)hipHostFree(c->df_host);
   c->df_host = nullptr;
   c->df_ready = false;
@@ -523,8 +527,11 @@ void enqueue_diff_backward(sf21_hii_ctx* c, const float* g, const float* emis, d
       launch_filter_i(n, c->spec, c->ubuf, kk, n3, c->wlut + (long)rb * c->mlut, c->mlut,
                       c->d_kappa + rb, c->tw, s, c->tile_filter, c->filter_w);
     });
-    launch(c, "cols_inv_j", [&] { launch_cols(n, n, 0, +1, c->ubuf, c->ubuf, kk, n3, c->tw, s, c->cols_tile); });
-    launch(c, "rows_phi", [&] { launch_rows_phi(n, c->ubuf, kk, n3, c->bal, c->df_st, inv_tau, c->tw, s); });
+    launch(c, "cols_inv_j", [&] { launch_cols(n, n, 0, +1, c->ubuf, c->ubuf, kk, n3, c->tw, s, c->cols_tile, c->cols_wide); });
+    launch(c, "rows_phi", [&] {
+      launch_rows_phi(n, c->ubuf, kk, n3, c->bal, c->df_st, inv_tau, c->tw,
+                      c->df_pairs ? c->df_pairs + (long)rb * n3 : nullptr, n3, s);
+    });
     launch(c, "cols_fwd_j", [&] { launch_cols(n, n, 0, -1, c->ubuf, c->ubuf, kp, n3, c->tw, s); });
     launch(c, "ball_acc_i", [&] {
       launch_ball_i(n, true, c->ubuf, n3, c->df_c, 0, kk, c->df_ball + (long)rb * h3, c->tw, s);
@@ -563,13 +570,16 @@ void enqueue_diff_backward(sf21_hii_ctx* c, const float* g, const float* emis, d
       launch_ball_i(n, false, c->df_c, 0, c->df_v, n3, kk, c->df_ball + (long)rb * h3, c->tw, s);
     });
     launch(c, "cols_inv_j", [&] { launch_cols(n, n, 0, +1, c->df_v, c->df_v, kp, n3, c->tw, s); });
-    launch(c, "filter_i", [&] {
-      launch_filter_i(n, c->spec, c->ubuf, kk, n3, c->wlut + (long)rb * c->mlut, c->mlut,
-                      c->d_kappa + rb, c->tw, s, c->tile_filter, c->filter_w);
-    });
-    launch(c, "cols_inv_j", [&] { launch_cols(n, n, 0, +1, c->ubuf, c->ubuf, kk, n3, c->tw, s, c->cols_tile); });
+    if (!c->df_pairs) {  // recompute the batch's filtered pairs
+      launch(c, "filter_i", [&] {
+        launch_filter_i(n, c->spec, c->ubuf, kk, n3, c->wlut + (long)rb * c->mlut, c->mlut,
+                        c->d_kappa + rb, c->tw, s, c->tile_filter, c->filter_w);
+      });
+      launch(c, "cols_inv_j", [&] { launch_cols(n, n, 0, +1, c->ubuf, c->ubuf, kk, n3, c->tw, s, c->cols_tile, c->cols_wide); });
+    }
     launch(c, "rows_seed", [&] {
-      launch_rows_seed(n, c->ubuf, c->df_v, kk, n3, n3, c->bal, c->df_st, inv_tau, c->df_rho, c->tw, s);
+      launch_rows_seed(n, c->ubuf, c->df_v, kk, n3, n3, c->bal, c->df_st, inv_tau, c->df_rho, c->tw,
+                       c->df_pairs ? c->df_pairs + (long)rb * n3 : nullptr, n3, s);
     });
     launch(c, "cols_fwd_j", [&] { launch_cols(n, n, 0, -1, c->ubuf, c->ubuf, kk, n3, c->tw, s); });
     launch(c, "filter_acc_i", [&] {
@@ -600,7 +610,7 @@ int diff_check(sf21_hii_ctx* c, const float* g, const float* emis, int nbins) {
 
 // ---- autotuner: choices of the fused path, persisted per (arch, n, radii) ----
 struct TuneChoice {
-  int tile = 16, K = 16, w = 1, ct = 16;
+  int tile = 16, K = 16, w = 1, ct = 16, cw = 0, rw = 0;
 };
 
 std::string tune_key(const sf21_hii_ctx* c) {
@@ -621,7 +631,8 @@ const char* tune_path(const char* given) {
   return (e && *e) ? e : nullptr;
 }
 
-// Cache lines: "<key> | tile=T K=K w=W ct=C | <ms> ms"; the last match wins.
+// Cache lines: "<key> | tile=T K=K w=W ct=C cw=0|1 rw=0|1 | <ms> ms"; the
+// last match wins.
 bool tune_lookup(const char* path, const std::string& key, TuneChoice* out) {
   FILE* f = path ? std::fopen(path, "r") : nullptr;
   if (!f) return false;
@@ -633,7 +644,8 @@ bool tune_lookup(const char* path, const std::string& key, TuneChoice* out) {
     std::string k(line, (size_t)(bar - line));
     while (!k.empty() && k.back() == ' ') k.pop_back();
     TuneChoice t;
-    if (k == key && std::sscanf(bar + 1, " tile=%d K=%d w=%d ct=%d", &t.tile, &t.K, &t.w, &t.ct) == 4) {
+    if (k == key && std::sscanf(bar + 1, " tile=%d K=%d w=%d ct=%d cw=%d rw=%d", &t.tile, &t.K,
+                                &t.w, &t.ct, &t.cw, &t.rw) == 6) {
       *out = t;
       found = true;
     }
@@ -645,7 +657,8 @@ bool tune_lookup(const char* path, const std::string& key, TuneChoice* out) {
 void tune_store(const char* path, const std::string& key, const TuneChoice& t, double ms) {
   FILE* f = path ? std::fopen(path, "a") : nullptr;
   if (!f) return;
-  std::fprintf(f, "%s | tile=%d K=%d w=%d ct=%d | %.3f ms\n", key.c_str(), t.tile, t.K, t.w, t.ct, ms);
+  std::fprintf(f, "%s | tile=%d K=%d w=%d ct=%d cw=%d rw=%d | %.3f ms\n", key.c_str(), t.tile,
+               t.K, t.w, t.ct, t.cw, t.rw, ms);
   std::fclose(f);
 }
 
@@ -681,10 +694,14 @@ int set_radius_batch(sf21_hii_ctx* c, int K) {
   return SF21_OK;
 }
 
+bool wide_ok(int n) { return n == 128 || n == 256 || n == 512; }
+
 void apply_choice(sf21_hii_ctx* c, const TuneChoice& t) {
   c->tile_filter = t.tile;
   c->filter_w = t.w;
   c->cols_tile = t.ct;
+  c->cols_wide = t.cw && wide_ok(c->n);
+  c->rows_wide = t.rw && wide_ok(c->n);
   drop_graphs(c);
 }
 
@@ -744,6 +761,8 @@ int sf21_hii_create(const sf21_hii_config* cfg, sf21_hii_ctx** out) {
     c->tile_filter = tc.tile;
     c->filter_w = tc.w;
     c->cols_tile = tc.ct;
+    c->cols_wide = tc.cw && wide_ok(c->n);
+    c->rows_wide = tc.rw && wide_ok(c->n);
   }
   if (c->K > (int)c->radii.size()) c->K = (int)c->radii.size();
 
@@ -1101,39 +1120,72 @@ int sf21_hii_autotune(sf21_hii_ctx* c, const float* a, const float* b, const flo
   }
   const bool prof = c->prof;
   c->prof = false;
-  TuneChoice best;
-  double best_t = 1e300;
+  // time one candidate: 1 warm-up + 2 timed slices
+  auto timed = [&](const TuneChoice& t, double* ms_out) -> int {
+    if (set_radius_batch(c, t.K)) return 1;  // does not fit: skip
+    apply_choice(c, t);
+    enqueue_fused(c, a, b, cc, z, xout, c->neutral_dev, -1, nullptr);
+    (void)hipEventRecord(e0, c->stream);
+    for (int r = 0; r < 2; ++r) enqueue_fused(c, a, b, cc, z, xout, c->neutral_dev, -1, nullptr);
+    (void)hipEventRecord(e1, c->stream);
+    float ms = 0.f;
+    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+      return -1;
+    *ms_out = ms / 2;
+    return 0;
+  };
+  // coordinate search from the defaults: each dimension in turn, the others
+  // held at the best so far
   const int nr = (int)c->radii.size();
-  std::vector<int> ks;
-  for (int k : {8, 12, 16})
-    if (std::find(ks.begin(), ks.end(), std::min(k, nr)) == ks.end()) ks.push_back(std::min(k, nr));
-  for (int K : ks) {
-    if (set_radius_batch(c, K)) continue;  // does not fit: skip
-    for (int w : {1, 0})
-      for (int tile : {16, 8})
-        for (int ct : {16, 8}) {
-          TuneChoice t;
-          t.tile = tile;
-          t.K = c->K;
-          t.w = w;
-          t.ct = ct;
-          apply_choice(c, t);
-          enqueue_fused(c, a, b, cc, z, xout, c->neutral_dev, -1, nullptr);  // warm
-          (void)hipEventRecord(e0, c->stream);
-          for (int r = 0; r < 2; ++r)
-            enqueue_fused(c, a, b, cc, z, xout, c->neutral_dev, -1, nullptr);
-          (void)hipEventRecord(e1, c->stream);
-          float ms = 0.f;
-          if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
-            rc = SF21_ERR_HIP;
-            break;
-          }
-          if (ms / 2 < best_t) {
-            best_t = ms / 2;
-            best = t;
-          }
-        }
-    if (rc) break;
+  TuneChoice best;
+  best.K = std::min(16, nr);
+  double best_t = 1e300;
+  {
+    double ms;
+    int r0 = timed(best, &ms);
+    if (r0 < 0) rc = SF21_ERR_HIP;
+    else if (r0 == 0) best_t = ms;
+  }
+  auto consider = [&](TuneChoice t) {
+    if (rc) return;
+    double ms;
+    int r1 = timed(t, &ms);
+    if (r1 < 0) rc = SF21_ERR_HIP;
+    else if (r1 == 0 && ms < best_t) {
+      best_t = ms;
+      best = t;
+    }
+  };
+  for (int K : {8, 12}) {
+    TuneChoice t = best;
+    t.K = std::min(K, nr);
+    if (t.K != best.K) consider(t);
+  }
+  {
+    TuneChoice t = best;
+    t.w = 0;
+    consider(t);
+  }
+  {
+    TuneChoice t = best;
+    t.tile = 8;
+    consider(t);
+  }
+  {
+    const TuneChoice base = best;
+    const int cols_opts[4][2] = {{8, 0}, {16, 0}, {16, 1}, {32, 1}};
+    for (auto& o : cols_opts) {
+      if (o[1] && !wide_ok(c->n)) continue;
+      TuneChoice t = base;
+      t.ct = o[0];
+      t.cw = o[1];
+      if (t.ct != base.ct || t.cw != base.cw) consider(t);
+    }
+  }
+  if (wide_ok(c->n)) {
+    TuneChoice t = best;
+    t.rw = 1;
+    consider(t);
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
@@ -1149,12 +1201,14 @@ int sf21_hii_autotune(sf21_hii_ctx* c, const float* a, const float* b, const flo
 }
 
 int sf21_hii_tuning(const sf21_hii_ctx* c, int* filter_tile, int* radius_batch, int* filter_w,
-                    int* cols_tile) {
+                    int* cols_tile, int* cols_wide, int* rows_wide) {
   if (!c) return SF21_ERR_ARG;
   if (filter_tile) *filter_tile = c->tile_filter;
   if (radius_batch) *radius_batch = c->K;
   if (filter_w) *filter_w = c->filter_w ? 1 : 2;
   if (cols_tile) *cols_tile = c->cols_tile;
+  if (cols_wide) *cols_wide = c->cols_wide ? 1 : 0;
+  if (rows_wide) *rows_wide = c->rows_wide ? 1 : 0;
   return SF21_OK;
 }
 
@@ -1336,6 +1390,18 @@ int sf21_fft_destroy(sf21_fft_plan* p) {
 
 
 // ---- differentiable mode ---------------------------------------------------
+int sf21_hii_diff_store_fields(sf21_hii_ctx* c, int enable) {
+  if (!c) return SF21_ERR_ARG;
+  if (hipSetDevice(c->cfg.device) != hipSuccess) return SF21_ERR_HIP;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (!enable) {
+    dev_free(c->df_pairs);
+    return SF21_OK;
+  }
+  if (c->df_pairs) return SF21_OK;
+  return dev_alloc(&c->df_pairs, (size_t)c->n3 * c->radii.size());
+}
+
 int sf21_hii_diff_forward_dev(sf21_hii_ctx* c, const float* g, const float* emis, double zeta,
                               int nbins, float* x_out, double* mean_x, double* pk) {
   int rc = diff_check(c, g, emis, nbins);

@@ -144,3 +144,23 @@ def test_relaxed_tends_to_exact(setup):
     _, _, mx, _ = s["ctx"].diff_backward(s["gd"], s["ed"], ZETA, 1e-3, 1.0, None, NB)
     print(f"\n<x~>(tau=1e-3) {mx:.6f}  exact <x> {mx_exact:.6f}", flush=True)
     assert abs(mx - mx_exact) < 2e-3
+
+
+def test_field_storage_mode_gives_the_same_gradient(setup):
+    """sf21_hii_diff_store_fields: the second sweep reads the first sweep's
+    filtered pairs instead of recomputing them; the same FP32 values, so the
+    gradient is identical bit for bit."""
+    s = setup
+    cp = _weights(s["ref"], s["g"])
+    g1, z1, m1, p1 = s["ctx"].diff_backward(s["gd"], s["ed"], ZETA, 0.02, 1.5, cp, NB)
+    g1 = g1.clone()
+    s["ctx"].diff_store_fields(True)
+    try:
+        g2, z2, m2, p2 = s["ctx"].diff_backward(s["gd"], s["ed"], ZETA, 0.02, 1.5, cp, NB)
+    finally:
+        s["ctx"].diff_store_fields(False)
+    assert torch.equal(g1, g2)
+    assert z1 == z2 and m1 == m2
+    # P_b's per-bin FP64 sums use LDS atomics: their order, hence the last
+    # bits, may differ from run to run
+    assert np.allclose(p1, p2, rtol=1e-12, atol=0)
