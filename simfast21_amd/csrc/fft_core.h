@@ -259,16 +259,21 @@ __device__ __forceinline__ int fold_sq(int i, int n) {
 // A plain FP32 x = kappa*sqrtf(m) carries ~x*6e-8 of phase error, i.e. a
 // relative W error of ~5e-5 at x ~ 700 (R = 100 Mpc at 512^3); the library
 // sincosf would also carry its large-argument reduction code.
-__device__ __forceinline__ float w_tophat_k(float2 kap, int m) {
+// sqrt(m) as s + s_lo (s_lo from the exact residual m - s^2): independent of
+// the radius, computed once per element.
+struct SqrtPair {
+  float s, s_lo;
+};
+__device__ __forceinline__ SqrtPair sqrt_pair(int m) {
   const float fm = (float)m;
   const float s = __builtin_amdgcn_sqrtf(fm);
-  const float s_lo = fmaf(-s, s, fm) * (0.5f * __builtin_amdgcn_rcpf(s));
-  const float p = kap.x * s;
-  const float x_lo = fmaf(kap.x, s, -p) + fmaf(kap.x, s_lo, kap.y * s);
+  return SqrtPair{s, fmaf(-s, s, fm) * (0.5f * __builtin_amdgcn_rcpf(s))};
+}
+
+__device__ __forceinline__ float w_tophat_sq(float2 kap, SqrtPair sq, int m) {
+  const float p = kap.x * sq.s;
+  const float x_lo = fmaf(kap.x, sq.s, -p) + fmaf(kap.x, sq.s_lo, kap.y * sq.s);
   const float x = p + x_lo;
-  const float q = x * x;
-  const float ser = 1.0f + q * (-0.1f + q * (1.0f / 280.0f + q * (-1.0f / 15120.0f +
-                    q * (1.0f / 1330560.0f + q * (-1.0f / 172972800.0f + q * (1.0f / 31135104000.0f))))));
   const float nq = rintf(p * 0.6366197466850281f);
   float r = fmaf(-nq, 1.5703125f, p);
   r = fmaf(-nq, 4.837512969970703e-4f, r);
@@ -284,8 +289,50 @@ __device__ __forceinline__ float w_tophat_k(float2 kap, int m) {
   const float sx = (qd & 2) ? -sn : sn;                   // sin(r + qd pi/2)
   const float cx = ((qd + 1) & 2) ? -cn : cn;             // cos(r + qd pi/2)
   const float inv = __builtin_amdgcn_rcpf(x);
-  const float big = 3.0f * (sx - x * cx) * (inv * inv * inv);
-  return m == 0 ? 1.0f : (x < 1.25f ? ser : big);
+  float w = 3.0f * (sx - x * cx) * (inv * inv * inv);
+  if (x < 1.25f) {  // rare (low |k|): skipped by whole waves
+    const float q = x * x;
+    w = 1.0f + q * (-0.1f + q * (1.0f / 280.0f + q * (-1.0f / 15120.0f +
+        q * (1.0f / 1330560.0f + q * (-1.0f / 172972800.0f + q * (1.0f / 31135104000.0f))))));
+  }
+  return m == 0 ? 1.0f : w;
+}
+
+__device__ __forceinline__ float w_tophat_k(float2 kap, int m) {
+  return w_tophat_sq(kap, sqrt_pair(m), m);
+}
+
+// Lanes of the i-axis passes.  W depends on |k|^2 only, so the line
+// positions k and L - k share it.  Thread slot q (threadIdx.x / TILE) holds
+// line position t with the pairs (t, TPL - t) in adjacent slots -- q = 2j +
+// i: j = 0 -> t = 0, TPL/2 (their mirrors are their own); j >= 1 -> t = j,
+// TPL - j -- so a thread's mirror partner is lane ^ TILE of the same wave.
+// Memory coalescing is unchanged (columns stay the fastest lane index).
+__device__ __forceinline__ int paired_slot_to_t(int q, int tpl) {
+  const int j = q >> 1, i = q & 1;
+  return j == 0 ? (i ? tpl / 2 : 0) : (i ? tpl - j : j);
+}
+
+// W for the E positions t + TPL m of a thread, evaluating only the lower
+// half (m <= E/2) and taking the upper half from the mirror partner:
+// position t + TPL m mirrors to (TPL - t) + TPL (E - 1 - m) for 0 < t < TPL,
+// to TPL (E - m) for t = 0 and to TPL/2 + TPL (E - 1 - m) for t = TPL/2.
+template <int L, int E, int TILE>
+__device__ __forceinline__ void w_line_shared(float2 kap, const SqrtPair (&sq)[E / 2 + 1],
+                                              const int (&mm)[E / 2 + 1], int t,
+                                              float (&wv)[E]) {
+  constexpr int TPL = L / E;
+  float wl[E / 2 + 1];
+#pragma unroll
+  for (int m = 0; m <= E / 2; ++m) wl[m] = w_tophat_sq(kap, sq[m], mm[m]);
+#pragma unroll
+  for (int m = 0; m < E / 2; ++m) wv[m] = wl[m];
+#pragma unroll
+  for (int m = E / 2; m < E; ++m) {
+    const float other = __shfl_xor(wl[E - 1 - m], TILE);
+    const float own0 = (m == E / 2) ? wl[E / 2] : wl[(E - m) < E / 2 ? (E - m) : 0];
+    wv[m] = t == 0 ? own0 : (t == TPL / 2 ? wl[E - 1 - m] : other);
+  }
 }
 
 }  // namespace sf21

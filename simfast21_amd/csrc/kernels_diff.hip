@@ -402,13 +402,21 @@ __global__ __launch_bounds__(TILE*(L / Plan<L>::E)) void k_filter_acc_i(
     const float2* __restrict__ kappa, float2* acc, const float2* __restrict__ tw) {
   constexpr int E = Plan<L>::E, TPL = L / E, NT = L / TILE;
   __shared__ float lds[TILE * 2 * LdsLine<L>::kFloats];
-  const int c = threadIdx.x % TILE, t = threadIdx.x / TILE;
+  // mirror pairs (t, TPL - t) in adjacent slots: lanes ^ TILE share W
+  const int c = threadIdx.x % TILE, t = paired_slot_to_t(threadIdx.x / TILE, TPL);
   const int kj = blockIdx.x / NT;
   const int kp = (blockIdx.x % NT) * TILE + c;
   const long off = (long)kj * L + kp;
   float* sre = lds + c * 2 * LdsLine<L>::kFloats;
   float* sim = sre + LdsLine<L>::kFloats;
   const int mjp = fold_sq(kj, L) + fold_sq(kp, L);
+  int mm[E / 2 + 1];
+  SqrtPair sq[E / 2 + 1];
+#pragma unroll
+  for (int m = 0; m <= E / 2; ++m) {
+    mm[m] = mjp + fold_sq((t + TPL * m) % L, L);
+    sq[m] = sqrt_pair(mm[m]);
+  }
   float2 z[E];
 #pragma unroll
   for (int m = 0; m < E; ++m) z[m] = make_float2(0.f, 0.f);
@@ -427,9 +435,7 @@ __global__ __launch_bounds__(TILE*(L / Plan<L>::E)) void k_filter_acc_i(
 #pragma unroll
       for (int m = 0; m < E; ++m) wv[m] = w[mjp + fold_sq(tl + TPL * m, L)];
     } else {
-      const float2 kr = kappa[r];
-#pragma unroll
-      for (int m = 0; m < E; ++m) wv[m] = w_tophat_k(kr, mjp + fold_sq(tl + TPL * m, L));
+      w_line_shared<L, E, TILE>(kappa[r], sq, mm, t, wv);
     }
 #pragma unroll
     for (int m = 0; m < E; ++m) {

@@ -222,7 +222,8 @@ __global__ __launch_bounds__(TILE*(L / Plan<L>::E)) void k_filter_i(
     const float2* __restrict__ tw) {
   constexpr int E = Plan<L>::E, TPL = L / E, NT = L / TILE;
   __shared__ float lds[TILE * 2 * LdsLine<L>::kFloats];
-  const int c = threadIdx.x % TILE, t = threadIdx.x / TILE;
+  // mirror pairs (t, TPL - t) in adjacent slots: lanes ^ TILE share W
+  const int c = threadIdx.x % TILE, t = paired_slot_to_t(threadIdx.x / TILE, TPL);
   const int kj = blockIdx.x / NT;
   const int kp = (blockIdx.x % NT) * TILE + c;
   // element e of this line sits at line[e * L * L] (32-bit offsets: L^3 < 2^31)
@@ -234,6 +235,14 @@ __global__ __launch_bounds__(TILE*(L / Plan<L>::E)) void k_filter_i(
   for (int m = 0; m < E; ++m) z[m] = src[(t + TPL * m) * (L * L)];
   line_fft<L, -1>(z, sre, sim, t, tw);
   const int mjp = fold_sq(kj, L) + fold_sq(kp, L);
+  // |k|^2 and its square root for the lower-half positions (radius-independent)
+  int mm[E / 2 + 1];
+  SqrtPair sq[E / 2 + 1];
+#pragma unroll
+  for (int m = 0; m <= E / 2; ++m) {
+    mm[m] = mjp + fold_sq((t + TPL * m) % L, L);
+    sq[m] = sqrt_pair(mm[m]);
+  }
 #pragma unroll 1
   for (int r = 0; r < nrad; ++r) {
     const int tl = opaque_lane(t);
@@ -246,12 +255,10 @@ __global__ __launch_bounds__(TILE*(L / Plan<L>::E)) void k_filter_i(
         v[m] = make_float2(z[m].x * wv, z[m].y * wv);
       }
     } else {
-      const float2 kr = kappa[r];  // dk * R as hi + lo
+      float wv[E];
+      w_line_shared<L, E, TILE>(kappa[r], sq, mm, t, wv);  // dk R as hi + lo
 #pragma unroll
-      for (int m = 0; m < E; ++m) {
-        const float wv = w_tophat_k(kr, mjp + fold_sq(tl + TPL * m, L));
-        v[m] = make_float2(z[m].x * wv, z[m].y * wv);
-      }
+      for (int m = 0; m < E; ++m) v[m] = make_float2(z[m].x * wv[m], z[m].y * wv[m]);
     }
     line_fft<L, +1>(v, sre, sim, tl, tw);
     float2* __restrict__ dst = u + (long)r * ustride + (long)kj * L + kp;
