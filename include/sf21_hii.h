@@ -52,6 +52,7 @@ extern "C" {
                                 any-centre flag read back each radius */
 
 typedef struct sf21_hii_ctx sf21_hii_ctx;
+typedef struct sf21_params sf21_params;
 
 typedef struct sf21_hii_config {
   int n;                 /* N_smoothed; supported: 64, 96, 128, 192, 256, 512 (32 fused only) */
@@ -129,6 +130,20 @@ int sf21_hii_run_slice_halo(sf21_hii_ctx *ctx, const float *delta, const float *
 int sf21_hii_power_spectrum_dev(sf21_hii_ctx *ctx, const float *field, int nbins, double *pk,
                                 double *kmean, double *counts);
 
+/* ---- 21-cm brightness temperature: the step after (t21.c:116-252) --------
+ * One redshift on device boxes (n^3 floats; the context supplies n, its FFT
+ * tables and scratch; n >= 64): the velocity gradient of delta along
+ * p->vel_comp (1 = i, 2 = j, other = p) in k-space, (k_axis/k)^2, scaled by
+ * -dD/dt/(H D) and clamped to [-0.5, 1]; t21 = 23 mK (1+delta) S/(1+dv)
+ * (0.7/h)(Ob h^2/0.02) sqrt((0.15/Om/h^2)(1+z)/10) (1 - xHII) in kelvin,
+ * with S = x/(1+x) (1 - Tcmb/TK), x = x_alpha + x_c, when the three spin
+ * boxes are given (the reference's z > Zminsfr with use_Lya_xrays = T), else
+ * S = 1.  Means in FP64: *mean_t21, and *mean_ts of Tcmb/(1 - S) (spin
+ * branch; 0 otherwise). */
+int sf21_t21_dev(sf21_hii_ctx *ctx, const sf21_params *p, double z, const float *delta,
+                 const float *xhii, const float *xc, const float *xalpha, const float *tk,
+                 float *out, double *mean_t21, double *mean_ts);
+
 /* ---- Differentiable mode -------------------------------------------------
  * No counterpart in the reference (SimFast21 has no gradients); it wraps the
  * bubble finder of get_HIIbubbles.c:277-479 for inference work.  Context:
@@ -148,14 +163,16 @@ int sf21_hii_power_spectrum_dev(sf21_hii_ctx *ctx, const float *field, int nbins
  * the union of balls becomes x~ = q + (1 - q)(1 - exp(-sum_R B_R * phi_R)),
  * B_R the exact ball, q the quick fill) with respect to g (grad_g, device
  * n^3) and zeta (*grad_zeta), plus the relaxed <x~> and P_b (host,
- * nullable).  Per-radius fields are recomputed in the backward unless
- * sf21_hii_diff_store_fields is on; tau -> 0 gives back the exact forward.
- * Sums in FP64. */
-/* Field-storage mode of the backward (default off): the first sweep keeps
- * every radius' filtered pair (n_R n^3 complex floats, 47 GB at 512^3 with 47
- * radii) and the second sweep reads them instead of recomputing them.
- * SF21_ERR_ALLOC if they do not fit; the gradient is identical. */
-int sf21_hii_diff_store_fields(sf21_hii_ctx *ctx, int enable);
+ * nullable).  Per-radius fields are recomputed in the backward (never
+ * stored by the forward; see sf21_hii_diff_store_fields); tau -> 0 gives
+ * back the exact forward.  Sums in FP64. */
+/* Field storage in the backward: the first sweep recomputes every radius'
+ * filtered pair from the forward spectrum; mode 1 keeps them (n_R n^3
+ * complex floats, 47 GB at 512^3 with 47 radii) for the second sweep, mode 0
+ * recomputes them there as well, mode 2 (default) keeps them when they take
+ * at most half of the free device memory.  The gradient is identical;
+ * SF21_ERR_ALLOC if mode 1 does not fit. */
+int sf21_hii_diff_store_fields(sf21_hii_ctx *ctx, int mode);
 int sf21_hii_diff_forward_dev(sf21_hii_ctx *ctx, const float *g, const float *emis, double zeta,
                               int nbins, float *x_out, double *mean_x, double *pk);
 int sf21_hii_diff_backward_dev(sf21_hii_ctx *ctx, const float *g, const float *emis,
@@ -223,7 +240,7 @@ int sf21_fft_destroy(sf21_fft_plan *plan);
 
 /* The parameters get_HIIbubbles reads from base_dir/simfast21.ini, parsed as
  * Input_variables.c:16-141 does, with its derived values. */
-typedef struct sf21_params {
+struct sf21_params {
   int nthreads;
   long n_halo, n;          /* N_halo, N_smoothed */
   double zmin, zmax, dz;   /* zmin snapped to zmax - dz*ceil((zmax-zmin)/dz) (:133) */
@@ -234,9 +251,21 @@ typedef struct sf21_params {
   int bubble_nbins;
   int use_sfr;             /* 0 unless use_SFR = T */
   double dx, dx_halo, dk;  /* box_len/n, box_len/n_halo, 2*pi/box_len (:127-129) */
-} sf21_params;
+  /* keys of the 21-cm step (t21.c) */
+  double omega_m, omega_b, omega_lambda;
+  int vel_comp;            /* Vel_comp: 1 = i, 2 = j, other = p axis of the gradient */
+  int use_lya_xrays;       /* 0 unless use_Lya_xrays = T */
+  double zmin_sfr;         /* Zminsfr snapped like zmin and raised to it (:137-139) */
+};
 /* msg receives the reference's message for SF21_ERR_IO / SF21_ERR_FORMAT. */
 int sf21_params_read(const char *base_dir, sf21_params *out, char *msg, int msg_cap);
+
+/* Background cosmology of the 21-cm step (auxiliary.c:224-303): H(z) in
+ * 1/s, the growth factor D(z) (D(0) = 1) and dD/dt in 1/s, in FP64 with the
+ * reference's step sizes. */
+double sf21_hubble_z(const sf21_params *p, double z);
+double sf21_growth(const sf21_params *p, double z);
+double sf21_dgrowth_dt(const sf21_params *p, double z);
 
 /* Radius ladder of get_HIIbubbles.c:192, :298-299, :397, :400-402, :461:
  * bfactor = 10^(log10(rmax/dx)/nbins), R divided by bfactor in FP64 while

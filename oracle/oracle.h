@@ -93,6 +93,19 @@ void orc_dilate_weighted_brute(int n, const int *w, unsigned char *out);
 void orc_power_spectrum(int n, double L, const double *field, int nbins,
                         double *pk, double *kmean, long *counts);
 
+/* ---- 21-cm step (t21.c) and its background cosmology (auxiliary.c) ---- */
+typedef struct orc_cosmo {
+  double hubble, omega_m, omega_b, omega_lambda;
+} orc_cosmo;
+double orc_hubble_z(const orc_cosmo *c, double z);            /* auxiliary.c:224-226 */
+double orc_growth(const orc_cosmo *c, double z);              /* auxiliary.c:242-266 */
+double orc_dgrowth_dt(const orc_cosmo *c, double z);          /* auxiliary.c:285-292 */
+/* t21.c:121-251 for one redshift (xc, xalpha, tk all NULL: TS >> Tcmb);
+ * out: the float box written to deltaTb/; *mean_ts only in the spin branch */
+void orc_t21(int n, const orc_cosmo *c, int vel_comp, double z, const float *delta,
+             const float *xhii, const float *xc, const float *xalpha, const float *tk,
+             float *out, double *mean_t21, double *mean_ts);
+
 #ifdef __cplusplus
 }
 #endif

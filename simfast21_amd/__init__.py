@@ -81,6 +81,12 @@ class Params(ctypes.Structure):
         ("dx", ctypes.c_double),
         ("dx_halo", ctypes.c_double),
         ("dk", ctypes.c_double),
+        ("omega_m", ctypes.c_double),
+        ("omega_b", ctypes.c_double),
+        ("omega_lambda", ctypes.c_double),
+        ("vel_comp", ctypes.c_int),
+        ("use_lya_xrays", ctypes.c_int),
+        ("zmin_sfr", ctypes.c_double),
     ]
 
     def as_dict(self) -> dict:
@@ -121,6 +127,7 @@ def _lib():
     L.sf21_hii_power_spectrum_dev.argtypes = [vp, vp, i, vp, vp, vp]
     L.sf21_hii_halo_emissivity_dev.argtypes = [vp, vp, ctypes.c_long, d, vp]
     L.sf21_hii_diff_store_fields.argtypes = [vp, i]
+    L.sf21_t21_dev.argtypes = [vp, ctypes.POINTER(Params), d, vp, vp, vp, vp, vp, vp, dp, dp]
     L.sf21_hii_diff_forward_dev.argtypes = [vp, vp, vp, d, i, vp, dp, vp]
     L.sf21_hii_diff_backward_dev.argtypes = [vp, vp, vp, d, d, i, d, vp, vp, dp, dp, vp]
     L.sf21_hii_run_slice_halo.argtypes = [vp, vp, vp, ctypes.c_long, d, vp, dp]
@@ -129,6 +136,9 @@ def _lib():
     L.sf21_fft_destroy.argtypes = [vp]
     L.sf21_params_read.argtypes = [ctypes.c_char_p, ctypes.POINTER(Params), ctypes.c_char_p, i]
     L.sf21_ladder.argtypes = [d, d, i, dp, i, ip]
+    for fn in ("sf21_hubble_z", "sf21_growth", "sf21_dgrowth_dt"):
+        getattr(L, fn).restype = d
+        getattr(L, fn).argtypes = [ctypes.POINTER(Params), d]
     L.sf21_ball_d2max.restype = ctypes.c_long
     L.sf21_ball_d2max.argtypes = [d, d]
     _LIB = L
@@ -159,6 +169,23 @@ def read_params(base_dir: str | os.PathLike) -> dict:
         err.message = msg.value.decode(errors="replace")
         raise err
     return p.as_dict()
+
+
+def make_params(**kw) -> Params:
+    """A Params record from keyword values (fields not given are 0)."""
+    p = Params()
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def cosmology(z: float, hubble: float, omega_m: float, omega_lambda: float,
+              omega_b: float = 0.04) -> tuple[float, float, float]:
+    """(H(z) [1/s], growth D(z), dD/dt [1/s]) of auxiliary.c:224-292."""
+    p = make_params(hubble=hubble, omega_m=omega_m, omega_lambda=omega_lambda, omega_b=omega_b)
+    L = _lib()
+    return (L.sf21_hubble_z(ctypes.byref(p), z), L.sf21_growth(ctypes.byref(p), z),
+            L.sf21_dgrowth_dt(ctypes.byref(p), z))
 
 
 def ladder(rmax: float, dx: float, nbins: int) -> tuple[list[float], int]:
@@ -279,10 +306,11 @@ class HiiContext:
                "sf21_hii_power_spectrum_dev")
         return pk, km, ct
 
-    def diff_store_fields(self, enable: bool = True):
-        """Keep the backward's per-radius filtered pairs between its two sweeps
-        (n_R n^3 complex floats) instead of recomputing them."""
-        _check(_lib().sf21_hii_diff_store_fields(self._h, int(enable)), "sf21_hii_diff_store_fields")
+    def diff_store_fields(self, mode: int = 1):
+        """Backward field storage: 1 keep the first sweep's per-radius pairs
+        (n_R n^3 complex floats) for the second, 0 recompute them, 2 keep them
+        if they fit in half the free device memory (the default)."""
+        _check(_lib().sf21_hii_diff_store_fields(self._h, int(mode)), "sf21_hii_diff_store_fields")
 
     def diff_forward(self, g, emis, zeta: float, nbins: int = 20, x_out=None):
         """Exact forward of the differentiable model on device boxes g, emis:
@@ -316,6 +344,21 @@ class HiiContext:
                                                  pk.ctypes.data),
                "sf21_hii_diff_backward_dev")
         return grad_out, gz.value, mx.value, pk
+
+    def t21(self, params, z: float, delta, xhii, xc=None, xalpha=None, tk=None, out=None):
+        """21-cm brightness temperature of one redshift (sf21_t21_dev) on device
+        boxes; params: a Params or a dict of its fields.  Returns (box, mean
+        t21, mean TS)."""
+        import torch
+        p = params if isinstance(params, Params) else make_params(**params)
+        if out is None:
+            out = torch.empty_like(delta)
+        mt, ms = ctypes.c_double(), ctypes.c_double()
+        torch.cuda.current_stream().synchronize()
+        _check(_lib().sf21_t21_dev(self._h, ctypes.byref(p), float(z), _ptr(delta), _ptr(xhii), _ptr(xc),
+                                   _ptr(xalpha), _ptr(tk), _ptr(out), ctypes.byref(mt), ctypes.byref(ms)),
+               "sf21_t21_dev")
+        return out, mt.value, ms.value
 
     def weight_map(self) -> np.ndarray:
         out = np.empty(self.n ** 3, np.int32)

@@ -1,5 +1,6 @@
-/* Host logic of the bubble finder that needs no GPU: parameter file
- * parsing, the radius ladder and the ball bound.  Part of libsf21hii. */
+/* Host logic that needs no GPU: parameter file parsing, the radius ladder,
+ * the ball bound and the background cosmology of the 21-cm step.  Part of
+ * libsf21hii. */
 #define _GNU_SOURCE
 #include <math.h>
 #include <stdio.h>
@@ -75,6 +76,15 @@ int sf21_params_read(const char *base_dir, sf21_params *p, char *msg, int msg_ca
       if (!strcmp(val, "T")) p->use_sfr = 1;
       if (!strcmp(val, "F")) p->use_sfr = 0;
     }
+    else if (!strcmp(key, "Vel_comp")) p->vel_comp = atoi(val);
+    else if (!strcmp(key, "omega_matter")) p->omega_m = atof(val);
+    else if (!strcmp(key, "omega_baryon")) p->omega_b = atof(val);
+    else if (!strcmp(key, "omega_lambda")) p->omega_lambda = atof(val);
+    else if (!strcmp(key, "Zminsfr")) p->zmin_sfr = atof(val);
+    else if (!strcmp(key, "use_Lya_xrays")) {
+      if (!strcmp(val, "T")) p->use_lya_xrays = 1;
+      if (!strcmp(val, "F")) p->use_lya_xrays = 0;
+    }
   }
   fclose(f);
   p->box_len = p->box_len * p->hubble;
@@ -83,10 +93,46 @@ int sf21_params_read(const char *base_dir, sf21_params *p, char *msg, int msg_ca
   p->dk = 2 * SF21_PI / p->box_len;
   p->dx_halo = p->n_halo ? p->box_len / p->n_halo : 0.0;
   p->dx = p->n ? p->box_len / p->n : 0.0;
-  if (p->dz != 0.0)
+  if (p->dz != 0.0) {
     p->zmin = p->zmax - p->dz * (double)ceil((p->zmax - p->zmin) / p->dz);
+    /* Input_variables.c:137-139 */
+    p->zmin_sfr = p->zmax - p->dz * (double)ceil((p->zmax - p->zmin_sfr) / p->dz);
+    if (p->zmin_sfr < p->zmin) p->zmin_sfr = p->zmin;
+  }
   set_msg(msg, msg_cap, "");
   return SF21_OK;
+}
+
+/* ---- background cosmology (auxiliary.c:224-303) ----
+ * H(z) = h H0 sqrt(Om (1+z)^3 + OL) in 1/s, H0 = 3.24078e-18 h/s
+ * (Input_variables.h:108); the growth factor is H(z)/H(0) I(z)/I(0) with
+ * I(z) the midpoint sum of (1+z')/(H(z')/H(0))^3 over [z, z + 100) in steps
+ * of 1e-3 (the loop variable accumulated in FP64 as the reference does);
+ * dD/dt the central difference of step 1e-3 in z times dz/dt = -H(z)(1+z). */
+#define SF21_H0 3.24078e-18
+
+double sf21_hubble_z(const sf21_params *p, double z) {
+  return p->hubble * SF21_H0 * sqrt(p->omega_m * (1. + z) * (1. + z) * (1. + z) + p->omega_lambda);
+}
+
+static double growth_integral(const sf21_params *p, double zz) {
+  const double dz = 0.001, h0 = sf21_hubble_z(p, 0.);
+  double s = 0., z, e;
+  for (z = zz + dz / 2.; z < zz + 100.; z += dz) {
+    e = sf21_hubble_z(p, z) / h0;
+    s += (1. + z) / e / e / e;
+  }
+  return s * dz;
+}
+
+double sf21_growth(const sf21_params *p, double z) {
+  return sf21_hubble_z(p, z) / sf21_hubble_z(p, 0.) * growth_integral(p, z) / growth_integral(p, 0.);
+}
+
+double sf21_dgrowth_dt(const sf21_params *p, double z) {
+  const double dz = 1e-3;
+  return (sf21_growth(p, z + dz / 2.) - sf21_growth(p, z - dz / 2.)) / dz *
+         (-sf21_hubble_z(p, z) * (1. + z));
 }
 
 int sf21_ladder(double rmax, double dx, int nbins, double *radii, int cap, int *n_main) {

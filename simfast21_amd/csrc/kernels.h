@@ -127,6 +127,13 @@ void launch_fill_c(float2* p, long count, hipStream_t s);
 void launch_fill_d(double* p, long count, hipStream_t s);
 void launch_to_complex(long n3, const float* in, float2* out, hipStream_t s);
 
+// ---- 21-cm brightness temperature ----
+int t21_partials_count(long n3);
+void launch_vgrad_filter(int n, float2* spec, int axis, hipStream_t s);
+void launch_t21_cells(long n3, const float* delta, const float* xhii, const float* dv_raw,
+                      const float* xc, const float* xa, const float* tk, float vfac, float c21,
+                      float tcmb, float* out, double* part, double* sums, hipStream_t s);
+
 // ---- halo-catalog emissivity ----
 void launch_halo_emissivity(long nhalos, const float* halos, int ng, int ns, int sm, float k_rion,
                             float c2, float c3, float scale, float* grid, float* out,

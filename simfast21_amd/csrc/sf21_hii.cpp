@@ -124,7 +124,11 @@ struct sf21_hii_ctx {
   double* df_coef = nullptr;
   double* df_host = nullptr;  // pinned staging
   float2* df_pairs = nullptr; // field-storage mode: n_R x n^3 filtered pairs
+  int df_store = 2;           // 0 recompute, 1 store, 2 store if it fits in half the free HBM
   bool df_ready = false;
+  // 21-cm step
+  float* t21_dv = nullptr;      // velocity gradient (unnormalised C2R output)
+  double* t21_part = nullptr;   // partial sums, then the two totals at the end
   // halo mode
   float* halo_grid = nullptr;   // N_halo^3 deposit grid
   float* halo_buf = nullptr;    // catalog upload (host entry point)
@@ -374,6 +378,7 @@ void free_all(sf21_hii_ctx* c) {
   dev_free(c->box_a); dev_free(c->box_b); dev_free(c->bubble);
   dev_free(c->spec2); dev_free(c->tmp2); dev_free(c->th); dev_free(c->flag_dev);
   dev_free(c->halo_grid); dev_free(c->halo_buf);
+  dev_free(c->t21_dv); dev_free(c->t21_part);
   dev_free(c->pk_bin); dev_free(c->pk_kabs); dev_free(c->pk_part); dev_free(c->pk_out);
   dev_free(c->df_delta); dev_free(c->df_rec); dev_free(c->df_ion); dev_free(c->df_x);
   dev_free(c->df_t); dev_free(c->df_fq); dev_free(c->df_fa); dev_free(c->df_qbar);
@@ -532,7 +537,7 @@ void enqueue_diff_backward(sf21_hii_ctx* c, const float* g, const float* emis, d
       launch_rows_phi(n, c->ubuf, kk, n3, c->bal, c->df_st, inv_tau, c->tw,
                       c->df_pairs ? c->df_pairs + (long)rb * n3 : nullptr, n3, s);
     });
-    launch(c, "cols_fwd_j", [&] { launch_cols(n, n, 0, -1, c->ubuf, c->ubuf, kp, n3, c->tw, s); });
+    launch(c, "cols_fwd_j", [&] { launch_cols(n, n, 0, -1, c->ubuf, c->ubuf, kp, n3, c->tw, s, c->cols_tile, c->cols_wide); });
     launch(c, "ball_acc_i", [&] {
       launch_ball_i(n, true, c->ubuf, n3, c->df_c, 0, kk, c->df_ball + (long)rb * h3, c->tw, s);
     });
@@ -569,7 +574,7 @@ void enqueue_diff_backward(sf21_hii_ctx* c, const float* g, const float* emis, d
     launch(c, "ball_i", [&] {
       launch_ball_i(n, false, c->df_c, 0, c->df_v, n3, kk, c->df_ball + (long)rb * h3, c->tw, s);
     });
-    launch(c, "cols_inv_j", [&] { launch_cols(n, n, 0, +1, c->df_v, c->df_v, kp, n3, c->tw, s); });
+    launch(c, "cols_inv_j", [&] { launch_cols(n, n, 0, +1, c->df_v, c->df_v, kp, n3, c->tw, s, c->cols_tile, c->cols_wide); });
     if (!c->df_pairs) {  // recompute the batch's filtered pairs
       launch(c, "filter_i", [&] {
         launch_filter_i(n, c->spec, c->ubuf, kk, n3, c->wlut + (long)rb * c->mlut, c->mlut,
@@ -581,7 +586,7 @@ void enqueue_diff_backward(sf21_hii_ctx* c, const float* g, const float* emis, d
       launch_rows_seed(n, c->ubuf, c->df_v, kk, n3, n3, c->bal, c->df_st, inv_tau, c->df_rho, c->tw,
                        c->df_pairs ? c->df_pairs + (long)rb * n3 : nullptr, n3, s);
     });
-    launch(c, "cols_fwd_j", [&] { launch_cols(n, n, 0, -1, c->ubuf, c->ubuf, kk, n3, c->tw, s); });
+    launch(c, "cols_fwd_j", [&] { launch_cols(n, n, 0, -1, c->ubuf, c->ubuf, kk, n3, c->tw, s, c->cols_tile, c->cols_wide); });
     launch(c, "filter_acc_i", [&] {
       launch_filter_acc_i(n, c->ubuf, n3, kk, c->wlut + (long)rb * c->mlut, c->mlut, c->d_kappa + rb,
                           c->filter_w, c->df_g, c->tw, s);
@@ -605,7 +610,18 @@ int diff_check(sf21_hii_ctx* c, const float* g, const float* emis, int nbins) {
   (void)hipGetLastError();
   int rc;
   if ((rc = ensure_pk(c, nbins))) return rc;
-  return ensure_diff(c);
+  if ((rc = ensure_diff(c))) return rc;
+  if (c->df_store == 2 && !c->df_pairs) {
+    // automatic field storage: keep the pairs when they take at most half
+    // of the free device memory
+    size_t free_b = 0, total_b = 0;
+    const size_t need = sizeof(float2) * (size_t)c->n3 * c->radii.size();
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && need <= free_b / 2 &&
+        dev_alloc(&c->df_pairs, (size_t)c->n3 * c->radii.size()) != SF21_OK)
+      c->df_pairs = nullptr;
+    (void)hipGetLastError();
+  }
+  return SF21_OK;
 }
 
 // ---- autotuner: choices of the fused path, persisted per (arch, n, radii) ----
@@ -1321,6 +1337,48 @@ int sf21_hii_run_slice_halo(sf21_hii_ctx* c, const float* delta, const float* ha
   return rc;
 }
 
+// ---- 21-cm brightness temperature (t21.c:116-252) -------------------------
+int sf21_t21_dev(sf21_hii_ctx* c, const sf21_params* p, double z, const float* delta,
+                 const float* xhii, const float* xc, const float* xalpha, const float* tk,
+                 float* out, double* mean_t21, double* mean_ts) {
+  if (!c || !p || !delta || !xhii || !out || c->n < 64) return SF21_ERR_ARG;
+  const bool spin = xc || xalpha || tk;
+  if (spin && !(xc && xalpha && tk)) return SF21_ERR_ARG;
+  if (!(p->hubble > 0) || !(p->omega_m > 0)) return SF21_ERR_ARG;
+  if (hipSetDevice(c->cfg.device) != hipSuccess) return SF21_ERR_HIP;
+  (void)hipGetLastError();
+  int rc;
+  const int n = c->n;
+  const long n3 = c->n3;
+  if (!c->t21_dv && ((rc = dev_alloc(&c->t21_dv, (size_t)n3)) ||
+                     (rc = dev_alloc(&c->t21_part, (size_t)2 * t21_partials_count(n3) + 2))))
+    return rc;
+  // per-redshift scalars in FP64 (auxiliary.c:224-292, t21.c:197-212)
+  const double growth = sf21_growth(p, z), dgdt = sf21_dgrowth_dt(p, z), hz = sf21_hubble_z(p, z);
+  const double h = p->hubble;
+  const double c21 = 23.0 / 1000. * (0.7 / h) * (p->omega_b * h * h / 0.02) *
+                     std::sqrt((0.15 / p->omega_m / h / h) * (1. + z) / 10.);
+  // -dD/dt / (H D) times the unnormalised transform's 1/N^3 (dx^3/L^3)
+  const double vfac = -dgdt / hz / growth / (double)n3;
+  const double tcmb = 2.725 * (1. + z);
+  r2c3d(c, delta, c->spec);
+  launch(c, "vgrad_filter", [&] { launch_vgrad_filter(n, c->spec, p->vel_comp, c->stream); });
+  c2r3d(c, c->spec, c->t21_dv);
+  double* sums = c->t21_part + 2 * t21_partials_count(n3);
+  launch(c, "t21_cells", [&] {
+    launch_t21_cells(n3, delta, xhii, c->t21_dv, xc, xalpha, tk, (float)vfac, (float)c21,
+                     (float)tcmb, out, c->t21_part, sums, c->stream);
+  });
+  double host[2];
+  if (hipMemcpyAsync(host, sums, sizeof(host), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess)
+    return SF21_ERR_HIP;
+  if (mean_t21) *mean_t21 = host[0] / (double)n3;
+  if (mean_ts) *mean_ts = spin ? host[1] / (double)n3 : 0.0;
+  if (c->prof) collect(c);
+  return check_hip();
+}
+
 // ---- FFTW-like transforms ------------------------------------------------
 int sf21_fft_plan_3d(int n, int kind, sf21_fft_plan** out) {
   if (!out || kind < SF21_FFT_R2C || kind > SF21_FFT_C2C_BACKWARD) return SF21_ERR_ARG;
@@ -1390,15 +1448,16 @@ int sf21_fft_destroy(sf21_fft_plan* p) {
 
 
 // ---- differentiable mode ---------------------------------------------------
-int sf21_hii_diff_store_fields(sf21_hii_ctx* c, int enable) {
-  if (!c) return SF21_ERR_ARG;
+int sf21_hii_diff_store_fields(sf21_hii_ctx* c, int mode) {
+  if (!c || mode < 0 || mode > 2) return SF21_ERR_ARG;
   if (hipSetDevice(c->cfg.device) != hipSuccess) return SF21_ERR_HIP;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  if (!enable) {
+  c->df_store = mode;
+  if (mode == 0) {
     dev_free(c->df_pairs);
     return SF21_OK;
   }
-  if (c->df_pairs) return SF21_OK;
+  if (c->df_pairs || mode == 2) return SF21_OK;  // automatic: decided at the next backward
   return dev_alloc(&c->df_pairs, (size_t)c->n3 * c->radii.size());
 }
 

@@ -20,6 +20,11 @@ class HiiParams(ctypes.Structure):
                 ("precision", ctypes.c_int), ("dilation", ctypes.c_int)]
 
 
+class Cosmo(ctypes.Structure):
+    _fields_ = [("hubble", ctypes.c_double), ("omega_m", ctypes.c_double),
+                ("omega_b", ctypes.c_double), ("omega_lambda", ctypes.c_double)]
+
+
 def lib():
     global _L
     if _L is None:
@@ -55,6 +60,11 @@ def lib():
         for nm in ("orc_r2c3d_d", "orc_c2r3d_d", "orc_r2c3d_f", "orc_c2r3d_f"):
             getattr(L, nm).argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
         L.orc_c2c3d_d.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_int]
+        for nm in ("orc_hubble_z", "orc_growth", "orc_dgrowth_dt"):
+            getattr(L, nm).restype = d
+            getattr(L, nm).argtypes = [ctypes.POINTER(Cosmo), d]
+        L.orc_t21.argtypes = [ctypes.c_int, ctypes.POINTER(Cosmo), ctypes.c_int, d] + [ctypes.c_void_p] * 6 + \
+            [ctypes.POINTER(d), ctypes.POINTER(d)]
         _L = L
     return _L
 
@@ -118,3 +128,31 @@ def ladder(rmax, dx, nbins):
     nm = ctypes.c_int()
     k = lib().orc_ladder(rmax, dx, nbins, _p(buf), cap, ctypes.byref(nm))
     return list(buf[:k]), nm.value
+
+
+def cosmo(hubble, omega_m, omega_b, omega_lambda):
+    return Cosmo(hubble, omega_m, omega_b, omega_lambda)
+
+
+def growth(c, z):
+    return lib().orc_growth(ctypes.byref(c), z)
+
+
+def dgrowth_dt(c, z):
+    return lib().orc_dgrowth_dt(ctypes.byref(c), z)
+
+
+def hubble_z(c, z):
+    return lib().orc_hubble_z(ctypes.byref(c), z)
+
+
+def t21(n, c, vel_comp, z, delta, xhii, xc=None, xalpha=None, tk=None):
+    """FP64 restatement of t21.c for one redshift: (box float32, mean, mean TS)."""
+    delta = np.ascontiguousarray(delta, np.float32).ravel()
+    xhii = np.ascontiguousarray(xhii, np.float32).ravel()
+    sp = [None if a is None else np.ascontiguousarray(a, np.float32).ravel() for a in (xc, xalpha, tk)]
+    out = np.zeros(n ** 3, np.float32)
+    mt, ms = ctypes.c_double(), ctypes.c_double()
+    lib().orc_t21(n, ctypes.byref(c), vel_comp, z, _p(delta), _p(xhii), _p(sp[0]), _p(sp[1]), _p(sp[2]),
+                  _p(out), ctypes.byref(mt), ctypes.byref(ms))
+    return out, mt.value, ms.value
