@@ -144,6 +144,24 @@ int sf21_t21_dev(sf21_hii_ctx *ctx, const sf21_params *p, double z, const float 
                  const float *xhii, const float *xc, const float *xalpha, const float *tk,
                  float *out, double *mean_t21, double *mean_ts);
 
+/* ---- The step before: the bubble finder's inputs -------------------------
+ * SFRD box of one redshift from a halo catalog (get_SFR.c:98-148; device
+ * pointers): CIC of sfr(M, z) = Rion(M, z)/Qion(z) (user_functions.c:71-74)
+ * on the N_halo^3 grid, smooth_box to n^3, divided by dx_halo^3 (Msun per
+ * year per (Mpc/h)^3).  The context needs n_halo >= n.  *mean (nullable):
+ * the box average in FP64 (the sfrd_av table). */
+int sf21_sfrd_dev(sf21_hii_ctx *ctx, const float *halos, long n_halos, double z, float *out,
+                  double *mean);
+/* Non-linear density of one redshift (get_nldensity.c:117-166; device
+ * pointers): each N_halo^3 cell carries mass 1 + delta0 D(300) to its
+ * position plus v N_halo h/L_h (D(z) - D(300)) (v: the z = 0 velocity boxes
+ * in Mpc) and deposits it by CIC on a grid that starts at -1; box_halo
+ * (nullable: context scratch) receives that N_halo^3 box and out (nullable)
+ * its smooth_box average to n^3 -- the deltanl box of the bubble finder. */
+int sf21_nldensity_dev(sf21_hii_ctx *ctx, const sf21_params *p, double z, const float *vx,
+                       const float *vy, const float *vz, const float *delta0, float *box_halo,
+                       float *out);
+
 /* ---- Differentiable mode -------------------------------------------------
  * No counterpart in the reference (SimFast21 has no gradients); it wraps the
  * bubble finder of get_HIIbubbles.c:277-479 for inference work.  Context:
@@ -256,6 +274,7 @@ struct sf21_params {
   int vel_comp;            /* Vel_comp: 1 = i, 2 = j, other = p axis of the gradient */
   int use_lya_xrays;       /* 0 unless use_Lya_xrays = T */
   double zmin_sfr;         /* Zminsfr snapped like zmin and raised to it (:137-139) */
+  int save_original_nl;    /* Original_nldensity_box = T: keep the N_halo^3 box too */
 };
 /* msg receives the reference's message for SF21_ERR_IO / SF21_ERR_FORMAT. */
 int sf21_params_read(const char *base_dir, sf21_params *out, char *msg, int msg_cap);

@@ -93,6 +93,13 @@ void orc_dilate_weighted_brute(int n, const int *w, unsigned char *out);
 void orc_power_spectrum(int n, double L, const double *field, int nbins,
                         double *pk, double *kmean, long *counts);
 
+/* ---- the bubble finder's inputs (get_SFR.c, get_nldensity.c) ---- */
+void orc_sfrd(long nhalos, const float *halos /* M,x,y,z */, double z, long nh, long n,
+              double dx_halo, float *sfrd, double *mean);           /* get_SFR.c:113-148 */
+void orc_nldensity(long nh, long n, double L_h, double hubble, double growth_in, double growth,
+                   const float *vx, const float *vy, const float *vz, const float *delta0,
+                   float *box_halo, float *out);                    /* get_nldensity.c:95-166 */
+
 /* ---- 21-cm step (t21.c) and its background cosmology (auxiliary.c) ---- */
 typedef struct orc_cosmo {
   double hubble, omega_m, omega_b, omega_lambda;

@@ -87,6 +87,7 @@ class Params(ctypes.Structure):
         ("vel_comp", ctypes.c_int),
         ("use_lya_xrays", ctypes.c_int),
         ("zmin_sfr", ctypes.c_double),
+        ("save_original_nl", ctypes.c_int),
     ]
 
     def as_dict(self) -> dict:
@@ -127,6 +128,8 @@ def _lib():
     L.sf21_hii_power_spectrum_dev.argtypes = [vp, vp, i, vp, vp, vp]
     L.sf21_hii_halo_emissivity_dev.argtypes = [vp, vp, ctypes.c_long, d, vp]
     L.sf21_hii_diff_store_fields.argtypes = [vp, i]
+    L.sf21_sfrd_dev.argtypes = [vp, vp, ctypes.c_long, d, vp, dp]
+    L.sf21_nldensity_dev.argtypes = [vp, ctypes.POINTER(Params), d, vp, vp, vp, vp, vp, vp]
     L.sf21_t21_dev.argtypes = [vp, ctypes.POINTER(Params), d, vp, vp, vp, vp, vp, vp, dp, dp]
     L.sf21_hii_diff_forward_dev.argtypes = [vp, vp, vp, d, i, vp, dp, vp]
     L.sf21_hii_diff_backward_dev.argtypes = [vp, vp, vp, d, d, i, d, vp, vp, dp, dp, vp]
@@ -344,6 +347,30 @@ class HiiContext:
                                                  pk.ctypes.data),
                "sf21_hii_diff_backward_dev")
         return grad_out, gz.value, mx.value, pk
+
+    def sfrd(self, halos, z: float, out=None):
+        """SFRD box of one redshift from a device halo catalog (n, 4):
+        (box, mean).  The context needs n_halo."""
+        import torch
+        if out is None:
+            out = torch.empty(self.n ** 3, dtype=torch.float32, device=halos.device)
+        mean = ctypes.c_double()
+        torch.cuda.current_stream().synchronize()
+        _check(_lib().sf21_sfrd_dev(self._h, _ptr(halos), int(halos.shape[0]), float(z), _ptr(out),
+                                    ctypes.byref(mean)), "sf21_sfrd_dev")
+        return out, mean.value
+
+    def nldensity(self, params, z: float, vx, vy, vz, delta0, box_halo=None, out=None):
+        """Non-linear density of one redshift (sf21_nldensity_dev): returns
+        (out (n^3), box_halo (N_halo^3) or None)."""
+        import torch
+        p = params if isinstance(params, Params) else make_params(**params)
+        if out is None:
+            out = torch.empty(self.n ** 3, dtype=torch.float32, device=vx.device)
+        torch.cuda.current_stream().synchronize()
+        _check(_lib().sf21_nldensity_dev(self._h, ctypes.byref(p), float(z), _ptr(vx), _ptr(vy), _ptr(vz),
+                                         _ptr(delta0), _ptr(box_halo), _ptr(out)), "sf21_nldensity_dev")
+        return out, box_halo
 
     def t21(self, params, z: float, delta, xhii, xc=None, xalpha=None, tk=None, out=None):
         """21-cm brightness temperature of one redshift (sf21_t21_dev) on device

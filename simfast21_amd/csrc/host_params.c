@@ -85,6 +85,10 @@ int sf21_params_read(const char *base_dir, sf21_params *p, char *msg, int msg_ca
       if (!strcmp(val, "T")) p->use_lya_xrays = 1;
       if (!strcmp(val, "F")) p->use_lya_xrays = 0;
     }
+    else if (!strcmp(key, "Original_nldensity_box")) {
+      if (!strcmp(val, "T")) p->save_original_nl = 1;
+      if (!strcmp(val, "F")) p->save_original_nl = 0;
+    }
   }
   fclose(f);
   p->box_len = p->box_len * p->hubble;

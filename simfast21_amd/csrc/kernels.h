@@ -134,7 +134,12 @@ void launch_t21_cells(long n3, const float* delta, const float* xhii, const floa
                       const float* xc, const float* xa, const float* tk, float vfac, float c21,
                       float tcmb, float* out, double* part, double* sums, hipStream_t s);
 
-// ---- halo-catalog emissivity ----
+// ---- halo-catalog emissivity, SFRD and non-linear density deposits ----
+int sum_partials_count(long count);
+void launch_sum_f32(long count, const float* x, double* part, double* out, hipStream_t s);
+void launch_nldensity(int ng, int ns, int sm, const float* vx, const float* vy, const float* vz,
+                      const float* delta0, float vscale, float growth_in, float growth,
+                      float* grid, float* out, hipStream_t s);
 void launch_halo_emissivity(long nhalos, const float* halos, int ng, int ns, int sm, float k_rion,
                             float c2, float c3, float scale, float* grid, float* out,
                             hipStream_t s);

@@ -132,6 +132,7 @@ struct sf21_hii_ctx {
   // halo mode
   float* halo_grid = nullptr;   // N_halo^3 deposit grid
   float* halo_buf = nullptr;    // catalog upload (host entry point)
+  double* sum_part = nullptr;   // FP64 partial sums of a box, then the total
   long halo_cap = 0;
   // profiling
   bool prof = false;
@@ -378,7 +379,7 @@ void free_all(sf21_hii_ctx* c) {
   dev_free(c->box_a); dev_free(c->box_b); dev_free(c->bubble);
   dev_free(c->spec2); dev_free(c->tmp2); dev_free(c->th); dev_free(c->flag_dev);
   dev_free(c->halo_grid); dev_free(c->halo_buf);
-  dev_free(c->t21_dv); dev_free(c->t21_part);
+  dev_free(c->t21_dv); dev_free(c->t21_part); dev_free(c->sum_part);
   dev_free(c->pk_bin); dev_free(c->pk_kabs); dev_free(c->pk_part); dev_free(c->pk_out);
   dev_free(c->df_delta); dev_free(c->df_rec); dev_free(c->df_ion); dev_free(c->df_x);
   dev_free(c->df_t); dev_free(c->df_fq); dev_free(c->df_fa); dev_free(c->df_qbar);
@@ -1378,6 +1379,100 @@ int sf21_t21_dev(sf21_hii_ctx* c, const sf21_params* p, double z, const float* d
   if (c->prof) collect(c);
   return check_hip();
 }
+
+}  // extern "C"
+
+// ---- the step before: SFRD and non-linear density deposits --------------
+namespace {
+// smooth_box's block size (auxiliary.c:654-656)
+long smooth_block(int ng, int ns) {
+  long sm = (long)roundf((1.0f * ng) / ns);
+  if ((long)ns * sm > ng) sm--;
+  return sm;
+}
+int ensure_halo_grid(sf21_hii_ctx* c) {
+  const long ng = c->cfg.n_halo;
+  if (c->halo_grid) return SF21_OK;
+  return dev_alloc(&c->halo_grid, (size_t)ng * ng * ng);
+}
+int box_mean(sf21_hii_ctx* c, const float* box, long count, double* mean) {
+  if (!c->sum_part) {
+    int rc = dev_alloc(&c->sum_part, (size_t)sum_partials_count(c->n3 > count ? c->n3 : count) + 1);
+    if (rc) return rc;
+  }
+  double* tot = c->sum_part + sum_partials_count(c->n3 > count ? c->n3 : count);
+  launch_sum_f32(count, box, c->sum_part, tot, c->stream);
+  double h = 0.0;
+  if (hipMemcpyAsync(&h, tot, sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess)
+    return SF21_ERR_HIP;
+  *mean = h / (double)count;
+  return SF21_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int sf21_sfrd_dev(sf21_hii_ctx* c, const float* halos, long nhalos, double z, float* out,
+                  double* mean) {
+  if (!c || (!halos && nhalos > 0) || nhalos < 0 || !out) return SF21_ERR_ARG;
+  if (c->cfg.n_halo < c->n || c->cfg.n_halo > 4096) return SF21_ERR_ARG;
+  if (hipSetDevice(c->cfg.device) != hipSuccess) return SF21_ERR_HIP;
+  (void)hipGetLastError();
+  int rc;
+  if ((rc = ensure_halo_grid(c))) return rc;
+  const int ng = c->cfg.n_halo, ns = c->n;
+  const long sm = smooth_block(ng, ns);
+  if (sm < 1) return SF21_ERR_ARG;
+  const double dx_h = c->L / ng;
+  // sfr = Rion/Qion (user_functions.c:71-74); Msun/yr per halo, then the
+  // block average divided by dx_halo^3 (get_SFR.c:126-139)
+  const float k_sfr = (float)(1.575e39 * std::pow(1.0 + z, 2.27044695e+00) / kVnorm / fit_Qion(z));
+  launch(c, "sfrd", [&] {
+    launch_halo_emissivity(nhalos, halos, ng, ns, (int)sm, k_sfr, (float)9.367608e+07, 0.44f,
+                           (float)(1.0 / (dx_h * dx_h * dx_h)), c->halo_grid, out, c->stream);
+  });
+  if (mean && (rc = box_mean(c, out, c->n3, mean))) return rc;
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return SF21_ERR_HIP;
+  if (c->prof) collect(c);
+  return check_hip();
+}
+
+int sf21_nldensity_dev(sf21_hii_ctx* c, const sf21_params* p, double z, const float* vx,
+                       const float* vy, const float* vz, const float* delta0, float* box_halo,
+                       float* out) {
+  if (!c || !p || !vx || !vy || !vz || !delta0 || (!out && !box_halo)) return SF21_ERR_ARG;
+  if (c->cfg.n_halo < c->n || c->cfg.n_halo > 4096 || !(p->box_len > 0) || !(p->hubble > 0))
+    return SF21_ERR_ARG;
+  if (hipSetDevice(c->cfg.device) != hipSuccess) return SF21_ERR_HIP;
+  (void)hipGetLastError();
+  int rc;
+  float* grid = box_halo;
+  if (!grid) {
+    if ((rc = ensure_halo_grid(c))) return rc;
+    grid = c->halo_grid;
+  }
+  const int ng = c->cfg.n_halo, ns = c->n;
+  const long sm = smooth_block(ng, ns);
+  if (sm < 1) return SF21_ERR_ARG;
+  // get_nldensity.c:95-99, :117, :132: velocities to cells (box_len in
+  // Mpc/h, the files in Mpc), the z = 0 mass 1 + delta0 D(300), the
+  // displacement factor D(z) - D(300)
+  const double growth_in = sf21_growth(p, 300.0);
+  const double growth = sf21_growth(p, z) - growth_in;
+  const double vscale = (double)ng / p->box_len * p->hubble;
+  launch(c, "nldensity", [&] {
+    launch_nldensity(ng, ns, (int)sm, vx, vy, vz, delta0, (float)vscale, (float)growth_in,
+                     (float)growth, grid, out, c->stream);
+  });
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return SF21_ERR_HIP;
+  if (c->prof) collect(c);
+  return check_hip();
+}
+
+}  // extern "C"
+
+extern "C" {
 
 // ---- FFTW-like transforms ------------------------------------------------
 int sf21_fft_plan_3d(int n, int kind, sf21_fft_plan** out) {

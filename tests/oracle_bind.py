@@ -63,6 +63,9 @@ def lib():
         for nm in ("orc_hubble_z", "orc_growth", "orc_dgrowth_dt"):
             getattr(L, nm).restype = d
             getattr(L, nm).argtypes = [ctypes.POINTER(Cosmo), d]
+        L.orc_sfrd.argtypes = [ctypes.c_long, ctypes.c_void_p, d, ctypes.c_long, ctypes.c_long, d,
+                               ctypes.c_void_p, ctypes.POINTER(d)]
+        L.orc_nldensity.argtypes = [ctypes.c_long, ctypes.c_long, d, d, d, d] + [ctypes.c_void_p] * 6
         L.orc_t21.argtypes = [ctypes.c_int, ctypes.POINTER(Cosmo), ctypes.c_int, d] + [ctypes.c_void_p] * 6 + \
             [ctypes.POINTER(d), ctypes.POINTER(d)]
         _L = L
@@ -156,3 +159,20 @@ def t21(n, c, vel_comp, z, delta, xhii, xc=None, xalpha=None, tk=None):
     lib().orc_t21(n, ctypes.byref(c), vel_comp, z, _p(delta), _p(xhii), _p(sp[0]), _p(sp[1]), _p(sp[2]),
                   _p(out), ctypes.byref(mt), ctypes.byref(ms))
     return out, mt.value, ms.value
+
+
+def sfrd(halos, z, nh, n, dx_halo):
+    halos = np.ascontiguousarray(halos, np.float32)
+    out = np.zeros(n ** 3, np.float32)
+    mean = ctypes.c_double()
+    lib().orc_sfrd(len(halos), _p(halos), z, nh, n, dx_halo, _p(out), ctypes.byref(mean))
+    return out, mean.value
+
+
+def nldensity(nh, n, L_h, hubble, growth_in, growth, vx, vy, vz, delta0):
+    a = [np.ascontiguousarray(x, np.float32).ravel() for x in (vx, vy, vz, delta0)]
+    box = np.zeros(nh ** 3, np.float32)
+    out = np.zeros(n ** 3, np.float32)
+    lib().orc_nldensity(nh, n, L_h, hubble, growth_in, growth, _p(a[0]), _p(a[1]), _p(a[2]), _p(a[3]),
+                        _p(box), _p(out))
+    return box, out
