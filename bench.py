@@ -159,7 +159,8 @@ def bench_uneven(torch, sf, synth, n_boxes=64, steps=3, seed=4000):
     }
 
 
-def bench_diff(torch, sf, n, side, radii, n_src, taus=(0.05, 0.02, 0.01), reps=3, seed=5005):
+def bench_diff(torch, sf, n, side, radii, n_src, taus=(0.05, 0.02, 0.01), reps=3, seed=5005,
+               tune=True):
     """Differentiable mode at one redshift: Gaussian g (sigma 1.2, P ~ k^-2.5),
     emissivity pattern from n_src sources ~ (1+delta)^2, zeta set for
     <x> ~ 0.5.  Times the exact forward (fields, bubble finder, T, P(k)) and
@@ -174,6 +175,11 @@ def bench_diff(torch, sf, n, side, radii, n_src, taus=(0.05, 0.02, 0.01), reps=3
     ctx = sf.HiiContext(n, side, radii, sf.INPUT_DIRECT, sf.PATH_FUSED)
     x = torch.empty_like(g)
     grad = torch.empty_like(g)
+    if tune:
+        e = torch.exp(g.double())
+        e /= e.mean()
+        ctx.autotune((e ** 2).float().contiguous(), (emis * 75.0).contiguous())
+        del e
     # zeta for <x> ~ 0.5 (bisection in log zeta on the exact forward)
     lo, hi = 1e-2, 1e6
     zeta, mx = 1.0, 0.0
@@ -189,6 +195,7 @@ def bench_diff(torch, sf, n, side, radii, n_src, taus=(0.05, 0.02, 0.01), reps=3
     rng = np.random.default_rng(7)
     _, _, pk = ctx.diff_forward(g, emis, zeta, 20, x_out=x)
     cp = rng.standard_normal(20) / np.where(pk > 0, pk, 1.0)
+    ctx.diff_store_fields(0)  # recompute in both sweeps
     ctx.diff_backward(g, emis, zeta, taus[0], 1.0, cp, 20, grad_out=grad)  # warm: ball tables
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -203,12 +210,12 @@ def bench_diff(torch, sf, n, side, radii, n_src, taus=(0.05, 0.02, 0.01), reps=3
             _, gz, mxt, _ = ctx.diff_backward(g, emis, zeta, tau, 1.0, cp, 20, grad_out=grad)
         torch.cuda.synchronize()
         bwd[str(tau)] = round((time.perf_counter() - t0) * 1e3 / reps, 2)
-        print(f"[bench] diff tau={tau}: backward {bwd[str(tau)]:.1f} ms, <x~> {mxt:.4f}, "
+        print(f"[bench] diff tau={tau}: backward (recompute) {bwd[str(tau)]:.1f} ms, <x~> {mxt:.4f}, "
               f"dL/dzeta {gz:.4e}", file=sys.stderr, flush=True)
-    # field-storage mode: the second sweep reads the first sweep's pairs
+    # default mode: the first sweep's pairs kept for the second when they fit
     bwd_stored = None
     try:
-        ctx.diff_store_fields(True)
+        ctx.diff_store_fields(1)
         bwd_stored = {}
         for tau in taus:
             ctx.diff_backward(g, emis, zeta, tau, 1.0, cp, 20, grad_out=grad)
@@ -218,10 +225,12 @@ def bench_diff(torch, sf, n, side, radii, n_src, taus=(0.05, 0.02, 0.01), reps=3
                 ctx.diff_backward(g, emis, zeta, tau, 1.0, cp, 20, grad_out=grad)
             torch.cuda.synchronize()
             bwd_stored[str(tau)] = round((time.perf_counter() - t0) * 1e3 / reps, 2)
-        ctx.diff_store_fields(False)
+        print(f"[bench] diff backward with stored fields {bwd_stored}", file=sys.stderr, flush=True)
     except sf.Sf21Error as e:  # does not fit
         bwd_stored = {"error": str(e)}
-    # per-kernel split of one backward
+    ctx.diff_store_fields(0)
+    ctx_tuning = ctx.tuning()
+    # per-kernel split of one backward (recompute mode)
     ctx.reset_stats()
     ctx.profile(True)
     ctx.diff_backward(g, emis, zeta, taus[0], 1.0, cp, 20, grad_out=grad)
@@ -238,6 +247,7 @@ def bench_diff(torch, sf, n, side, radii, n_src, taus=(0.05, 0.02, 0.01), reps=3
         "grad_over_forward": {k: round(v / fwd_ms, 3) for k, v in bwd.items()},
         "grad_over_forward_max": round(worst / fwd_ms, 3),
         "backward_ms_fields_stored": bwd_stored,
+        "tuning": ctx_tuning,
         "grad_over_forward_fields_stored": ({k: round(v / fwd_ms, 3) for k, v in bwd_stored.items()}
                                             if bwd_stored and "error" not in bwd_stored else None),
         "backward_kernels_ms": {k: [c, round(t, 3)] for k, (c, t) in kst.items()},
@@ -413,7 +423,7 @@ def main():
     diff = None
     if not args.no_diff:
         try:
-            diff = bench_diff(torch, sf, n, side, radii, n_src)
+            diff = bench_diff(torch, sf, n, side, radii, n_src, tune=not args.no_tune)
         finally:
             torch.cuda.empty_cache()
         print(f"[bench] diff forward {diff['forward_ms']:.1f} ms, grad/forward "
