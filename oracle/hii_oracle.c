@@ -281,6 +281,39 @@ void orc_dilate_weighted_brute(int n, const int *w, unsigned char *out) {
   }
 }
 
+/* Same result as orc_dilate_weighted_brute, by one squared EDT per
+ * distinct bound: cells with w[y] == D dilate every x with min-image
+ * |x-y|^2 <= D.  O(N^3) per distinct bound, for full-size boxes. */
+void orc_dilate_weighted_edt(int n, const int *w, unsigned char *out) {
+  const long n3 = (long)n * n * n;
+  long i;
+  int *vals = NULL, nv = 0, cap = 0;
+  unsigned char *mask = (unsigned char *)malloc((size_t)n3);
+  double *dist = (double *)malloc(sizeof(double) * n3);
+  memset(out, 0, (size_t)n3);
+  for (i = 0; i < n3; i++) {
+    int k, seen = 0;
+    if (w[i] < 0) continue;
+    for (k = 0; k < nv; k++)
+      if (vals[k] == w[i]) { seen = 1; break; }
+    if (seen) continue;
+    if (nv == cap) {
+      cap = cap ? 2 * cap : 64;
+      vals = (int *)realloc(vals, sizeof(int) * cap);
+    }
+    vals[nv++] = w[i];
+  }
+  for (int k = 0; k < nv; k++) {
+    for (i = 0; i < n3; i++) mask[i] = w[i] == vals[k];
+    edt3d_periodic(n, mask, dist);
+    for (i = 0; i < n3; i++)
+      if (dist[i] <= (double)vals[k]) out[i] = 1;
+  }
+  free(vals);
+  free(mask);
+  free(dist);
+}
+
 /* ------------------------------------------------------------------ */
 /* Per-slice pipeline, once in the reference's typing (float fields,   */
 /* FP32 FFT, FP64 W and ratios) and once in FP64 throughout.           */

@@ -87,6 +87,9 @@ void orc_filtered_f(int n, double dk, double R, const float *rec, const float *i
  * with w[y] >= 0 has min-image |x-y|^2 <= w[y].  Brute force over centres;
  * meant for small boxes in tests. */
 void orc_dilate_weighted_brute(int n, const int *w, unsigned char *out);
+/* The same map by one squared Euclidean distance transform per distinct
+ * bound (Felzenszwalb-Huttenlocher lower envelopes); for full-size boxes. */
+void orc_dilate_weighted_edt(int n, const int *w, unsigned char *out);
 
 /* 20-bin spherically averaged power spectrum of a real box (FP64), linear
  * bins in |k| index space: see orc_power_spectrum in hii_oracle.c. */

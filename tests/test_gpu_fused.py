@@ -47,26 +47,20 @@ def run_case(torch, case, path=sf.PATH_FUSED, radius_batch=0, filter_w=0):
     return x.cpu().numpy(), neu, w, ctx
 
 
-@pytest.mark.gpu
-@pytest.mark.parametrize("n,zeta,filter_w", [(64, 0.3, 0), (64, 3.0, 0), (128, 1.0, 0),
-                                             (64, 3.0, 2), (128, 1.0, 2)])
-def test_fused_vs_oracle(torch_cuda, n, zeta, filter_w):
-    """filter_w 0: top-hat weights evaluated in FP32 in the kernel (the
-    default); 2: the FP64 table per integer |k|^2."""
-    torch = torch_cuda
-    case = direct_case(n, zeta)
-    side, radii, dx = case["side"], case["radii"], case["side"] / n
+def check_vs_oracle(torch, case, filter_w=0, dilate=orc.dilate_brute, label=""):
+    """The parity bar above for one case; returns (gpu, oracle) mean x_HII."""
+    n, side, radii, dx = case["n"], case["side"], case["radii"], case["side"] / case["n"]
     x_g, neu_g, w_g, _ = run_case(torch, case, filter_w=filter_w)
     x_o, neu_o, fc_o = orc.hii_slice(n, side, radii, case["rec"], case["ion"], precision=64, dilation=1)
     # 1) exact dilation of the GPU's own map + quick fill
-    dil = orc.dilate_brute(n, w_g).astype(bool)
+    dil = dilate(n, w_g).astype(bool)
     expect = np.where(dil, np.float32(1.0), quick_fill_np(case["rec"].ravel(), case["ion"].ravel()))
     assert np.array_equal(x_g, expect)
     # 2) centre-map differences are near-threshold cells only
     w_o = weights_from_first_centre(fc_o, radii, dx)
     diff = np.nonzero(w_g != w_o)[0]
-    print(f"n={n} zeta={zeta} filter_w={filter_w}: centre-map mismatches {diff.size} of {n**3}, "
-          f"mean x_HII gpu {1 - neu_g:.7f} oracle {1 - neu_o:.7f}")
+    print(f"{label}: centre-map mismatches {diff.size} of {n**3}, "
+          f"mean x_HII gpu {1 - neu_g:.7f} oracle {1 - neu_o:.7f}", flush=True)
     if diff.size:
         d2 = np.array([orc.d2max(R, dx) for R in radii])
         groups = {}
@@ -99,16 +93,66 @@ def test_fused_vs_oracle(torch_cuda, n, zeta, filter_w):
                 assert ok[flip].all(), (k, cells[flip & ~ok][:5], (ii[cells] / rr[cells])[flip & ~ok][:5])
                 explained |= flip
                 print(f"  radius {k} R={radii[k]:.3f}: {int(flip.sum())} flipped of {cells.size}, "
-                      f"rel err rec {rel_rec:.2e} ion {rel_ion:.2e}")
+                      f"rel err rec {rel_rec:.2e} ion {rel_ion:.2e}", flush=True)
             assert explained.all(), cells[~explained][:5]
+        ctx.close()
     # 3) mean: within 1e-5, or fully attributed to the explained flips (each
     # cell whose dilation status differs moves the sum of 1 - x by at most 1;
     # at 64^3-128^3 one flipped small-R void centre alone is ~1e-5)
-    dil_o = orc.dilate_brute(n, w_o).astype(bool)
+    dil_o = dilate(n, w_o).astype(bool)
     D = int(np.count_nonzero(dil != dil_o))
-    print(f"  mean |diff| {abs(neu_g - neu_o):.2e}; cells with different dilation {D} ({D / n**3:.2e})")
+    print(f"  mean |diff| {abs(neu_g - neu_o):.2e}; cells with different dilation {D} ({D / n**3:.2e})",
+          flush=True)
     assert D / n ** 3 < 1e-4
     assert abs(neu_g - neu_o) <= max(1e-5, D / n ** 3 + 1e-12)
+    return 1 - neu_g, 1 - neu_o
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,zeta,filter_w", [(64, 0.3, 0), (64, 3.0, 0), (128, 1.0, 0),
+                                             (64, 3.0, 2), (128, 1.0, 2)])
+def test_fused_vs_oracle(torch_cuda, n, zeta, filter_w):
+    """filter_w 0: top-hat weights evaluated in FP32 in the kernel (the
+    default); 2: the FP64 table per integer |k|^2."""
+    check_vs_oracle(torch_cuda, direct_case(n, zeta), filter_w,
+                    label=f"n={n} zeta={zeta} filter_w={filter_w}")
+
+
+def zeta_for_mean(torch, case, target, tol=0.01):
+    """Efficiency scale giving mean x_HII within tol of target on the fused
+    path (bisection in log zeta; x_HII is non-decreasing in zeta)."""
+    n, side, radii = case["n"], case["side"], case["radii"]
+    ctx = sf.HiiContext(n, side, radii, sf.INPUT_DIRECT, sf.PATH_FUSED)
+    a = torch.from_numpy(case["rec"].ravel()).cuda()
+    e = torch.from_numpy(case["emis"].ravel()).cuda()
+    lo, hi = -8.0, 12.0
+    for _ in range(40):
+        mid = 0.5 * (lo + hi)
+        _, neu = ctx.run_slice(a, (e * float(np.exp(mid))).contiguous())
+        if abs(1 - neu - target) <= tol:
+            break
+        lo, hi = (mid, hi) if 1 - neu < target else (lo, mid)
+    ctx.close()
+    return float(np.exp(mid))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("target", [0.2, 0.5, 0.8])
+def test_fused_256_vs_oracle(torch_cuda, target):
+    """The 256^3 case at full size (side 300 Mpc, sigma 1.2, 30000 sources,
+    radii 80 Mpc by 0.9 to the cell size) at three ionised fractions, under
+    the same bar as the small cases, against the FP64 oracle with exact
+    dilation (distance transforms in place of brute force)."""
+    torch = torch_cuda
+    n, side = 256, 300.0
+    delta, emis, rec = sf.synth.direct_boxes(n, side, 30000, 2560)
+    case = dict(n=n, side=side, radii=sf.synth.geometric_ladder(80.0, 0.9, side / n), rec=rec,
+                emis=emis)
+    zeta = zeta_for_mean(torch, case, target)
+    case["ion"] = (emis * np.float32(zeta)).astype(np.float32)
+    xg, xo = check_vs_oracle(torch, case, dilate=orc.dilate_edt,
+                             label=f"n=256 target {target} zeta={zeta:.4g}")
+    assert abs(xg - target) <= 0.011
 
 
 @pytest.mark.gpu

@@ -112,6 +112,23 @@ def test_lattice_ball_counts():
         assert orc.dilate_brute(n, w).sum() == cnt
 
 
+@pytest.mark.parametrize("n,seed", [(16, 1), (20, 2), (24, 3)])
+def test_edt_dilation_equals_brute_force(n, seed):
+    """The distance-transform dilation used at full size equals brute force
+    over centres on random weighted maps, bounds up to past the half box
+    (periodic images) and a centre on every face."""
+    rng = np.random.default_rng(seed)
+    w = np.full(n ** 3, -1, np.int32)
+    cells = rng.choice(n ** 3, 12, replace=False)
+    w[cells] = rng.choice([0, 1, 2, 3, 5, 9, 14, 30, (n // 2) ** 2, 3 * (n // 2) ** 2], 12)
+    for face in (0, n - 1):
+        w[(face * n + n // 3) * n + 2] = 4
+        w[(3 * n + face) * n + n // 2] = 6
+        w[(n // 2 * n + 1) * n + face] = 2
+    assert np.array_equal(orc.dilate_edt(n, w), orc.dilate_brute(n, w))
+    assert orc.dilate_edt(n, np.full(n ** 3, -1, np.int32)).sum() == 0
+
+
 def test_oracle_uniform_known_answers():
     """Uniform fields: every filtered field equals the input (W(0) = 1, no
     other mode).  ion >= rec everywhere ionises the box; ion = c*rec with
