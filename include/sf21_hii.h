@@ -214,8 +214,10 @@ int sf21_hii_filtered_dev(sf21_hii_ctx *ctx, const float *a, const float *b, con
  * 16/12/8 (capped by the ladder; skipped when it does not fit), top-hat
  * weights in the kernel or from the FP64 table, filter tile 16/8, the
  * per-radius column passes (8 values per thread with 8 or 16 columns per
- * block, or 16 values per thread with 16 or 32 columns; the latter for
- * n = 128, 256, 512) and the centre-test pass (8 or 16 values per thread).
+ * block, or 16 values per thread with 16 or 32 columns, the latter for
+ * n = 128, 256, 512; with 16 or 32 columns also as blocks that walk 8 tiles
+ * and prefetch the next) and the centre-test pass (8 or 16 values per
+ * thread).
  * Keeps the fastest in the context and appends it to the cache file
  * (cache_path, else $SF21_TUNE_CACHE, else not recorded) keyed by
  * (device arch, n, number of radii).  A context created with radius_batch,
@@ -224,8 +226,9 @@ int sf21_hii_filtered_dev(sf21_hii_ctx *ctx, const float *a, const float *b, con
 int sf21_hii_autotune(sf21_hii_ctx *ctx, const float *a, const float *b, const float *c,
                       double z, const char *cache_path, double *best_ms);
 /* Current choices: filter tile, radius batch, top-hat weights (1 in the
- * kernel, 2 table), column-pass tile, 16 values per thread in the column
- * passes (0/1) and in the centre-test pass (0/1); any pointer may be NULL. */
+ * kernel, 2 table), column-pass tile, column-pass variant (bit 0: 16 values
+ * per thread, bit 1: prefetching blocks), 16 values per thread in the
+ * centre-test pass (0/1); any pointer may be NULL. */
 int sf21_hii_tuning(const sf21_hii_ctx *ctx, int *filter_tile, int *radius_batch, int *filter_w,
                     int *cols_tile, int *cols_wide, int *rows_wide);
 

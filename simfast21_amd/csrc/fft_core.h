@@ -131,6 +131,7 @@ __device__ __forceinline__ int opaque_lane(int v) {
 // LDS index padding: one slot every 8 breaks the stride-R write pattern of
 // the first stage.
 __device__ __forceinline__ int lds_pad(int i) { return i + (i >> 3); }
+__device__ __forceinline__ unsigned lds_pad_u(unsigned i) { return i + (i >> 3); }
 template <int L>
 struct LdsLine {
   static constexpr int kFloats = L + L / 8 + 1;  // odd stride between lines
@@ -161,8 +162,21 @@ template <> struct Radices<192> { using type = RadixList<4, 4, 4, 3>; };
 template <> struct Radices<256> { using type = RadixList<8, 8, 4>; };
 template <> struct Radices<512> { using type = RadixList<8, 8, 8>; };
 
+// LDS slot offset of r*NS past a write base: the padding of base + r*NS
+// equals pad(base) + r*NS + (r*NS >> 3) whenever the low three bits of base
+// cannot carry into the next multiple of 8 -- NS a multiple of 8, or NS | 8
+// with base = k (mod 8), k < NS (the base's other term a multiple of 8).
+template <int NS, int R>
+struct StageSlots {
+  static constexpr bool kConst = (NS % 8 == 0) || ((NS * R) % 8 == 0 && 8 % NS == 0);
+  static constexpr int off(int r) { return r * NS + ((r * NS) >> 3); }
+};
+
 // One Stockham stage; recurses into the remaining radices.
 // tw: table exp(-2*pi*i*q/L), q < L.  sre/sim: this line's LDS region.
+// Index arithmetic is unsigned (lanes are non-negative), and where the
+// thread's butterflies b share k = t mod NS (TPL a multiple of NS) the LDS
+// slots of every b and r are one per-thread base plus compile-time offsets.
 template <int L, int E, int SIGN, int NS, int R, int... REST>
 __device__ __forceinline__ void stages(float2 (&v)[E], float* sre, float* sim, int t,
                                        const float2* __restrict__ tw) {
@@ -170,17 +184,23 @@ __device__ __forceinline__ void stages(float2 (&v)[E], float* sre, float* sim, i
   constexpr int NB = E / R;
   constexpr bool LAST = sizeof...(REST) == 0;
   static_assert(E % R == 0, "radix must divide the per-thread count");
+  constexpr bool kUniformK = TPL % NS == 0;
+  constexpr bool kFastW = kUniformK && StageSlots<NS, R>::kConst && (TPL * R) % 8 == 0;
+  const unsigned ut = (unsigned)t;
+  const unsigned k0 = ut % NS;
+  const unsigned pbase0 = lds_pad_u((ut / NS) * NS * R + k0);
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
-    const int j = t + TPL * b;
-    const int k = j % NS;
+    const unsigned j = ut + TPL * b;
+    const unsigned k = kUniformK ? k0 : j % NS;
     float2 a[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) a[r] = v[b + r * NB];
     if constexpr (NS > 1) {
+      const unsigned ks = k * (unsigned)(L / (NS * R));
 #pragma unroll
       for (int r = 1; r < R; ++r) {
-        float2 w = tw[k * r * (L / (NS * R))];
+        float2 w = tw[ks * r];
         if (SIGN > 0) w.y = -w.y;
         a[r] = cmul(a[r], w);
       }
@@ -189,11 +209,19 @@ __device__ __forceinline__ void stages(float2 (&v)[E], float* sre, float* sim, i
     if constexpr (LAST) {
 #pragma unroll
       for (int r = 0; r < R; ++r) v[b + r * NB] = a[r];
-    } else {
-      const int base = (j / NS) * NS * R + k;
+    } else if constexpr (kFastW) {
+      const unsigned pb = pbase0 + (unsigned)(b * (TPL * R + (TPL * R) / 8));
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        const int o = lds_pad(base + r * NS);
+        const unsigned o = pb + StageSlots<NS, R>::off(r);
+        sre[o] = a[r].x;
+        sim[o] = a[r].y;
+      }
+    } else {
+      const unsigned base = (j / NS) * NS * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const unsigned o = lds_pad_u(base + r * NS);
         sre[o] = a[r].x;
         sim[o] = a[r].y;
       }
@@ -202,9 +230,10 @@ __device__ __forceinline__ void stages(float2 (&v)[E], float* sre, float* sim, i
   if constexpr (!LAST) {
     lds_pin();
     __syncthreads();
+    const unsigned pt = lds_pad_u(ut);
 #pragma unroll
     for (int m = 0; m < E; ++m) {
-      const int o = lds_pad(t + TPL * m);
+      const unsigned o = (TPL % 8 == 0) ? pt + (unsigned)(m * (TPL + TPL / 8)) : lds_pad_u(ut + TPL * m);
       v[m] = make_float2(sre[o], sim[o]);
     }
     lds_pin();

@@ -26,11 +26,12 @@ void launch_balance(long n3, const float* a, const float* b, const float* c, Cel
                     double* part, float* scale, hipStream_t s);
 void launch_fwd_rows(int n, const float* a, const float* b, const float* c, CellConsts k,
                      const float* bal, float2* y, const float2* tw, hipStream_t s);
-// tile: columns per block (8, 16; 16 or 32 when wide); wide: 16 values per
-// thread instead of the size's default (sizes 128, 256, 512)
+// tile: columns per block (8, 16; 16 or 32 when wide); variant bit 0: 16
+// values per thread instead of the size's default (sizes 128, 256, 512);
+// bit 1: blocks walk 8 tiles, prefetching the next (tile 16, or 32 wide)
 void launch_cols(int n, int width, int axis, int sign, const float2* in, float2* out,
                  int nbatch, long bstride, const float2* tw, hipStream_t s, int tile = 16,
-                 bool wide = false);
+                 int variant = 0);
 void launch_filter_i(int n, const float2* spec, float2* u, int nrad, long ustride,
                      const float* wlut, long m_lut, const float2* kappa, const float2* tw,
                      hipStream_t s, int tile, int wmode);

@@ -209,6 +209,64 @@ __global__ __launch_bounds__(TILE*(L / E)) void k_cols(const float2* in, float2*
   }
 }
 
+// The same pass with each block walking `per_block` consecutive tiles (tile
+// q: batch q / (ntile L), outer (q / ntile) mod L, column tile q mod ntile)
+// and loading tile q + 1 into a second register set before transforming and
+// storing tile q, so that one resident block per CU keeps loads in flight
+// while it computes.  Tiles are disjoint, so the pass may run in place.
+template <int L, int SIGN, int TILE, int E>
+__global__ __launch_bounds__(TILE*(L / E)) void k_cols_pipe(const float2* in, float2* out,
+                                                            int width, int axis, long bstride,
+                                                            const float2* __restrict__ tw,
+                                                            long ntiles, int per_block) {
+  constexpr int TPL = L / E;
+  __shared__ float lds[TILE * 2 * LdsLine<L>::kFloats];
+  const int c = threadIdx.x % TILE;
+  const int ntile = (width + TILE - 1) / TILE;
+  const long q0 = (long)blockIdx.x * per_block;
+  const long q1 = q0 + per_block < ntiles ? q0 + per_block : ntiles;
+  float* sre = lds + c * 2 * LdsLine<L>::kFloats;
+  float* sim = sre + LdsLine<L>::kFloats;
+  const long stride = axis == 0 ? (long)width : (long)L * width;
+  // first element of column c of tile q, -1 past the last column
+  auto origin = [&](long q) -> long {
+    const long y = q / ((long)ntile * L);
+    const long rem = q - y * ((long)ntile * L);
+    const int outer = (int)(rem / ntile);
+    const int col = (int)(rem % ntile) * TILE + c;
+    if (col >= width) return -1;
+    const long b = axis == 0 ? (long)outer * L * width + col : (long)outer * width + col;
+    return b + y * bstride;
+  };
+  float2 cur[E], nxt[E];
+  long cb = origin(q0);
+  {
+    const int t = (int)threadIdx.x / TILE;
+#pragma unroll
+    for (int m = 0; m < E; ++m)
+      cur[m] = cb >= 0 ? in[cb + (long)(t + TPL * m) * stride] : make_float2(0.f, 0.f);
+  }
+#pragma unroll 1
+  for (long q = q0; q < q1; ++q) {
+    const int t = opaque_lane((int)threadIdx.x / TILE);
+    long nb = -1;
+    if (q + 1 < q1) {
+      nb = origin(q + 1);
+#pragma unroll
+      for (int m = 0; m < E; ++m)
+        nxt[m] = nb >= 0 ? in[nb + (long)(t + TPL * m) * stride] : make_float2(0.f, 0.f);
+    }
+    line_fft_e<L, E, SIGN>(cur, sre, sim, t, tw);
+    if (cb >= 0) {
+#pragma unroll
+      for (int m = 0; m < E; ++m) out[cb + (long)(t + TPL * m) * stride] = cur[m];
+    }
+#pragma unroll
+    for (int m = 0; m < E; ++m) cur[m] = nxt[m];
+    cb = nb;
+  }
+}
+
 // ------------------------------------------------------------------------
 // Per radius batch: forward along i of the (p, j)-transformed spectrum,
 // top-hat filter W(|k| R) for each radius (get_HIIbubbles.c:305-322; W from
@@ -588,8 +646,20 @@ void launch_fwd_rows(int n, const float* a, const float* b, const float* c, Cell
 
 template <int L, int TILE, int E = Plan<L>::E>
 static void cols_t(int width, int axis, int sign, const float2* in, float2* out, int nbatch,
-                   long bstride, const float2* tw, hipStream_t s) {
+                   long bstride, const float2* tw, hipStream_t s, bool pipe) {
   constexpr int NT = TILE * (L / E);
+  if (pipe) {
+    constexpr int kPer = 8;
+    const long ntiles = (long)L * ((width + TILE - 1) / TILE) * nbatch;
+    const dim3 g((unsigned)((ntiles + kPer - 1) / kPer));
+    if (sign < 0)
+      hipLaunchKernelGGL((k_cols_pipe<L, -1, TILE, E>), g, dim3(NT), 0, s, in, out, width, axis,
+                         bstride, tw, ntiles, kPer);
+    else
+      hipLaunchKernelGGL((k_cols_pipe<L, +1, TILE, E>), g, dim3(NT), 0, s, in, out, width, axis,
+                         bstride, tw, ntiles, kPer);
+    return;
+  }
   dim3 g(L * ((width + TILE - 1) / TILE), nbatch);
   if (sign < 0)
     hipLaunchKernelGGL((k_cols<L, -1, TILE, E>), g, dim3(NT), 0, s, in, out, width, axis, bstride, tw);
@@ -597,24 +667,27 @@ static void cols_t(int width, int axis, int sign, const float2* in, float2* out,
     hipLaunchKernelGGL((k_cols<L, +1, TILE, E>), g, dim3(NT), 0, s, in, out, width, axis, bstride, tw);
 }
 
-// tile: columns per block, 16 (default) or 8; wide: 16 values per thread
-// (sizes 128-512; tile 16 or 32)
+// tile: columns per block, 16 (default) or 8; variant bit 0: 16 values per
+// thread (sizes 128-512; tile 16 or 32); bit 1: blocks walk 8 tiles with the
+// next tile's loads issued before the current tile's transform (tile 16, or
+// 32 with bit 0)
 void launch_cols(int n, int width, int axis, int sign, const float2* in, float2* out,
-                 int nbatch, long bstride, const float2* tw, hipStream_t s, int tile, bool wide) {
+                 int nbatch, long bstride, const float2* tw, hipStream_t s, int tile, int variant) {
+  const bool wide = variant & 1, pipe = (variant & 2) != 0;
 #define X(L)                                                                              \
   if (n == L) {                                                                           \
     if (tile == 8)                                                                        \
-      cols_t<L, 8>(width, axis, sign, in, out, nbatch, bstride, tw, s);                   \
+      cols_t<L, 8>(width, axis, sign, in, out, nbatch, bstride, tw, s, false);            \
     else                                                                                  \
-      cols_t<L, 16>(width, axis, sign, in, out, nbatch, bstride, tw, s);                  \
+      cols_t<L, 16>(width, axis, sign, in, out, nbatch, bstride, tw, s, pipe);            \
     return;                                                                               \
   }
 #define XW(L)                                                                             \
   if (n == L && wide) {                                                                   \
     if (tile == 32)                                                                       \
-      cols_t<L, 32, 16>(width, axis, sign, in, out, nbatch, bstride, tw, s);              \
+      cols_t<L, 32, 16>(width, axis, sign, in, out, nbatch, bstride, tw, s, pipe);        \
     else                                                                                  \
-      cols_t<L, 16, 16>(width, axis, sign, in, out, nbatch, bstride, tw, s);              \
+      cols_t<L, 16, 16>(width, axis, sign, in, out, nbatch, bstride, tw, s, false);       \
     return;                                                                               \
   }
   XW(128) XW(256) XW(512)

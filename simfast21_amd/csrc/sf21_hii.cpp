@@ -78,7 +78,7 @@ struct sf21_hii_ctx {
   int n = 0, h = 0, K = 0;
   int tile_filter = 16;
   int cols_tile = 16;  // columns per block of the per-radius passes along j
-  bool cols_wide = false;  // 16 values per thread in the per-radius passes along j
+  int cols_wide = 0;  // per-radius passes along j: bit 0 16 values per thread, bit 1 prefetching blocks
   bool rows_wide = false;  // 16 values per thread in the centre-test pass
   int filter_w = 1;  // 1: FP32 on the fly, 0: FP64 table per |k|^2
   float2* d_kappa = nullptr;  // dk R per radius as a float pair (hi, lo)
@@ -648,7 +648,7 @@ const char* tune_path(const char* given) {
   return (e && *e) ? e : nullptr;
 }
 
-// Cache lines: "<key> | tile=T K=K w=W ct=C cw=0|1 rw=0|1 | <ms> ms"; the
+// Cache lines: "<key> | tile=T K=K w=W ct=C cw=0..3 rw=0|1 | <ms> ms"; the
 // last match wins.
 bool tune_lookup(const char* path, const std::string& key, TuneChoice* out) {
   FILE* f = path ? std::fopen(path, "r") : nullptr;
@@ -713,11 +713,20 @@ int set_radius_batch(sf21_hii_ctx* c, int K) {
 
 bool wide_ok(int n) { return n == 128 || n == 256 || n == 512; }
 
+// Column-pass variant that exists for this size and tile: 16 values per
+// thread only for wide_ok sizes (tile 16 or 32), prefetching blocks only
+// with tile 16 (8 values) or tile 32 (16 values).
+int cols_variant(int n, int tile, int cw) {
+  const bool wide = (cw & 1) && wide_ok(n) && (tile == 16 || tile == 32);
+  const bool pipe = (cw & 2) && (wide ? tile == 32 : tile == 16);
+  return (wide ? 1 : 0) | (pipe ? 2 : 0);
+}
+
 void apply_choice(sf21_hii_ctx* c, const TuneChoice& t) {
   c->tile_filter = t.tile;
   c->filter_w = t.w;
   c->cols_tile = t.ct;
-  c->cols_wide = t.cw && wide_ok(c->n);
+  c->cols_wide = cols_variant(c->n, t.ct, t.cw);
   c->rows_wide = t.rw && wide_ok(c->n);
   drop_graphs(c);
 }
@@ -778,7 +787,7 @@ int sf21_hii_create(const sf21_hii_config* cfg, sf21_hii_ctx** out) {
     c->tile_filter = tc.tile;
     c->filter_w = tc.w;
     c->cols_tile = tc.ct;
-    c->cols_wide = tc.cw && wide_ok(c->n);
+    c->cols_wide = cols_variant(c->n, tc.ct, tc.cw);
     c->rows_wide = tc.rw && wide_ok(c->n);
   }
   if (c->K > (int)c->radii.size()) c->K = (int)c->radii.size();
@@ -1190,9 +1199,9 @@ int sf21_hii_autotune(sf21_hii_ctx* c, const float* a, const float* b, const flo
   }
   {
     const TuneChoice base = best;
-    const int cols_opts[4][2] = {{8, 0}, {16, 0}, {16, 1}, {32, 1}};
+    const int cols_opts[6][2] = {{8, 0}, {16, 0}, {16, 2}, {16, 1}, {32, 1}, {32, 3}};
     for (auto& o : cols_opts) {
-      if (o[1] && !wide_ok(c->n)) continue;
+      if ((o[1] & 1) && !wide_ok(c->n)) continue;
       TuneChoice t = base;
       t.ct = o[0];
       t.cw = o[1];
@@ -1224,7 +1233,7 @@ int sf21_hii_tuning(const sf21_hii_ctx* c, int* filter_tile, int* radius_batch, 
   if (radius_batch) *radius_batch = c->K;
   if (filter_w) *filter_w = c->filter_w ? 1 : 2;
   if (cols_tile) *cols_tile = c->cols_tile;
-  if (cols_wide) *cols_wide = c->cols_wide ? 1 : 0;
+  if (cols_wide) *cols_wide = c->cols_wide;
   if (rows_wide) *rows_wide = c->rows_wide ? 1 : 0;
   return SF21_OK;
 }

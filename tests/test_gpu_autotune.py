@@ -27,7 +27,7 @@ def test_autotune_records_and_reloads(torch_cuda, tmp_path):
     print(f"\nautotune {n}^3, {len(radii)} radii: {ms:.3f} ms/slice with {t}", flush=True)
     assert ms > 0
     assert t["filter_tile"] in (8, 16) and t["filter_w"] in (1, 2)
-    assert (t["cols_tile"], t["cols_wide"]) in ((8, 0), (16, 0), (16, 1), (32, 1))
+    assert (t["cols_tile"], t["cols_wide"]) in ((8, 0), (16, 0), (16, 2), (16, 1), (32, 1), (32, 3))
     assert t["radius_batch"] in (8, 12, 16, len(radii))
     lines = cache.read_text().strip().splitlines()
     assert len(lines) == 1 and f"n={n} radii={len(radii)}" in lines[0]
@@ -55,11 +55,13 @@ def test_autotune_records_and_reloads(torch_cuda, tmp_path):
             os.environ["SF21_TUNE_CACHE"] = old
 
 
-@pytest.mark.parametrize("cols,rows_wide", [((16, 1), 1), ((32, 1), 0), ((8, 0), 1)])
+@pytest.mark.parametrize("cols,rows_wide", [((16, 1), 1), ((32, 1), 0), ((8, 0), 1), ((16, 2), 0),
+                                            ((32, 3), 1)])
 def test_wide_variants_are_bit_identical(torch_cuda, tmp_path, cols, rows_wide):
-    """16 values per thread run the same butterflies on the same values as 8:
-    the fused result is identical bit for bit (choices forced through a
-    cache line for this context's key)."""
+    """16 values per thread run the same butterflies on the same values as 8,
+    and the prefetching column blocks (cw bit 1) the same transforms on the
+    same tiles: the fused result is identical bit for bit (choices forced
+    through a cache line for this context's key)."""
     torch = torch_cuda
     case = direct_case(128, zeta=1.0, seed=8200)
     n, side, radii = case["n"], case["side"], case["radii"]

@@ -137,22 +137,30 @@ def zeta_for_mean(torch, case, target, tol=0.01):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("target", [0.2, 0.5, 0.8])
+@pytest.mark.parametrize("target", [None, 0.5, 0.8])
 def test_fused_256_vs_oracle(torch_cuda, target):
     """The 256^3 case at full size (side 300 Mpc, sigma 1.2, 30000 sources,
-    radii 80 Mpc by 0.9 to the cell size) at three ionised fractions, under
-    the same bar as the small cases, against the FP64 oracle with exact
-    dilation (distance transforms in place of brute force)."""
+    radii 80 Mpc by 0.9 to the cell size) at mean x_HII 0.5 and 0.8 and at
+    the recipe's floor, under the same bar as the small cases, against the
+    FP64 oracle with exact dilation (distance transforms in place of brute
+    force).  The floor (target None, zeta = 1e-3): a cell whose filtered
+    emissivity is positive while the k-space top hat's ringing leaves its
+    filtered recombination field non-positive is a centre
+    (get_HIIbubbles.c:338-347); on this recipe such centres ionise ~1/3 of
+    the box at any efficiency, so a mean x_HII of 0.2 is out of reach."""
     torch = torch_cuda
     n, side = 256, 300.0
     delta, emis, rec = sf.synth.direct_boxes(n, side, 30000, 2560)
     case = dict(n=n, side=side, radii=sf.synth.geometric_ladder(80.0, 0.9, side / n), rec=rec,
                 emis=emis)
-    zeta = zeta_for_mean(torch, case, target)
+    zeta = 1e-3 if target is None else zeta_for_mean(torch, case, target)
     case["ion"] = (emis * np.float32(zeta)).astype(np.float32)
     xg, xo = check_vs_oracle(torch, case, dilate=orc.dilate_edt,
                              label=f"n=256 target {target} zeta={zeta:.4g}")
-    assert abs(xg - target) <= 0.011
+    if target is None:
+        assert 0.25 < xg < 0.45
+    else:
+        assert abs(xg - target) <= 0.011
 
 
 @pytest.mark.gpu
