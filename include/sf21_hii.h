@@ -212,7 +212,8 @@ int sf21_hii_filtered_dev(sf21_hii_ctx *ctx, const float *a, const float *b, con
  * are tuned by FFTW_MEASURE at get_HIIbubbles.c:110-183).  Times the slice
  * (a, b, c, z; device pointers) by a coordinate search over: radius batch
  * 16/12/8 (capped by the ladder; skipped when it does not fit), top-hat
- * weights in the kernel or from the FP64 table, filter tile 16/8, the
+ * weights in the kernel or from the FP64 table, filter tile 16/8, filter
+ * blocks over single i-lines or mirror pairs (kj, n - kj) sharing W, the
  * per-radius column passes (8 values per thread with 8 or 16 columns per
  * block, or 16 values per thread with 16 or 32 columns, the latter for
  * n = 128, 256, 512; with 16 or 32 columns also as blocks that walk 8 tiles
@@ -228,9 +229,10 @@ int sf21_hii_autotune(sf21_hii_ctx *ctx, const float *a, const float *b, const f
 /* Current choices: filter tile, radius batch, top-hat weights (1 in the
  * kernel, 2 table), column-pass tile, column-pass variant (bit 0: 16 values
  * per thread, bit 1: prefetching blocks), 16 values per thread in the
- * centre-test pass (0/1); any pointer may be NULL. */
+ * centre-test pass (0/1), filter blocks over mirror pairs of i-lines that
+ * share the weights (0/1); any pointer may be NULL. */
 int sf21_hii_tuning(const sf21_hii_ctx *ctx, int *filter_tile, int *radius_batch, int *filter_w,
-                    int *cols_tile, int *cols_wide, int *rows_wide);
+                    int *cols_tile, int *cols_wide, int *rows_wide, int *filter_pair);
 
 /* Effective radii per fused filter pass (the automatic choice depends on
  * free device memory); negative on error. */

@@ -122,7 +122,7 @@ def _lib():
     L.sf21_hii_profile.argtypes = [vp, i]
     L.sf21_hii_radius_batch.argtypes = [vp]
     L.sf21_hii_autotune.argtypes = [vp, vp, vp, vp, d, ctypes.c_char_p, dp]
-    L.sf21_hii_tuning.argtypes = [vp, ip, ip, ip, ip, ip, ip]
+    L.sf21_hii_tuning.argtypes = [vp, ip, ip, ip, ip, ip, ip, ip]
     L.sf21_hii_kernel_stats.argtypes = [vp, i, ctypes.c_char_p, i, ctypes.POINTER(ctypes.c_long), dp]
     L.sf21_hii_reset_stats.argtypes = [vp]
     L.sf21_hii_power_spectrum_dev.argtypes = [vp, vp, i, vp, vp, vp]
@@ -412,10 +412,11 @@ class HiiContext:
         return ms.value
 
     def tuning(self) -> dict:
-        v = [ctypes.c_int() for _ in range(6)]
+        v = [ctypes.c_int() for _ in range(7)]
         _check(_lib().sf21_hii_tuning(self._h, *[ctypes.byref(x) for x in v]), "sf21_hii_tuning")
         return {"filter_tile": v[0].value, "radius_batch": v[1].value, "filter_w": v[2].value,
-                "cols_tile": v[3].value, "cols_wide": v[4].value, "rows_wide": v[5].value}
+                "cols_tile": v[3].value, "cols_wide": v[4].value, "rows_wide": v[5].value,
+                "filter_pair": v[6].value}
 
     @property
     def radius_batch(self) -> int:

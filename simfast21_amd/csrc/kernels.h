@@ -32,9 +32,11 @@ void launch_fwd_rows(int n, const float* a, const float* b, const float* c, Cell
 void launch_cols(int n, int width, int axis, int sign, const float2* in, float2* out,
                  int nbatch, long bstride, const float2* tw, hipStream_t s, int tile = 16,
                  int variant = 0);
+// jpair: one block per mirror pair of i-lines (kj, n - kj), sharing W
+// (tile 8 or 16, weights evaluated in the kernel)
 void launch_filter_i(int n, const float2* spec, float2* u, int nrad, long ustride,
                      const float* wlut, long m_lut, const float2* kappa, const float2* tw,
-                     hipStream_t s, int tile, int wmode);
+                     hipStream_t s, int tile, int wmode, bool jpair = false);
 void launch_inv_rows_crit(int n, const float2* v, int nrad, long vstride, const int* d2max,
                           const float* bal, int* wmap, float2* dbg, hipStream_t s,
                           const float2* tw, bool wide = false);

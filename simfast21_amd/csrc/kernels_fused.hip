@@ -273,26 +273,40 @@ __global__ __launch_bounds__(TILE*(L / E)) void k_cols_pipe(const float2* in, fl
 // an FP64 table indexed by the integer |k|^2, rounded to float), inverse
 // along i.  The forward spectrum along i never leaves the registers.
 // ------------------------------------------------------------------------
-template <int L, int TILE, bool WTAB>
+// JPAIR: a block takes the lines at kj and L - kj (kj = 1 + block row, up
+// to L/2 - 1) of the same p columns, which share every W value (|k|^2
+// depends on kj^2 only): one W evaluation serves two lines, whose inverse
+// transforms run one after the other through the same LDS region.  Without
+// JPAIR a block takes the line kj = block row * kj_step (kj_step L/2 covers
+// the unpaired lines 0 and L/2).
+template <int L, int TILE, bool WTAB, bool JPAIR = false>
 __global__ __launch_bounds__(TILE*(L / Plan<L>::E)) void k_filter_i(
     const float2* __restrict__ spec, float2* __restrict__ u, int nrad, long ustride,
     const float* __restrict__ wlut, long mlut, const float2* __restrict__ kappa,
-    const float2* __restrict__ tw) {
+    const float2* __restrict__ tw, int kj_step) {
   constexpr int E = Plan<L>::E, TPL = L / E, NT = L / TILE;
+  constexpr int NL = JPAIR ? 2 : 1;
   __shared__ float lds[TILE * 2 * LdsLine<L>::kFloats];
   // mirror pairs (t, TPL - t) in adjacent slots: lanes ^ TILE share W
   const int c = threadIdx.x % TILE, t = paired_slot_to_t(threadIdx.x / TILE, TPL);
-  const int kj = blockIdx.x / NT;
+  const int row = blockIdx.x / NT;
+  int kjs[NL];
+  kjs[0] = JPAIR ? 1 + row : row * kj_step;
+  if constexpr (JPAIR) kjs[1] = L - 1 - row;
   const int kp = (blockIdx.x % NT) * TILE + c;
-  // element e of this line sits at line[e * L * L] (32-bit offsets: L^3 < 2^31)
-  const float2* __restrict__ src = spec + (long)kj * L + kp;
   float* sre = lds + c * 2 * LdsLine<L>::kFloats;
   float* sim = sre + LdsLine<L>::kFloats;
-  float2 z[E];
+  float2 z[NL][E];
 #pragma unroll
-  for (int m = 0; m < E; ++m) z[m] = src[(t + TPL * m) * (L * L)];
-  line_fft<L, -1>(z, sre, sim, t, tw);
-  const int mjp = fold_sq(kj, L) + fold_sq(kp, L);
+  for (int l = 0; l < NL; ++l) {
+    // element e of this line sits at line[e * L * L] (32-bit offsets: L^3 < 2^31)
+    const float2* __restrict__ src = spec + (long)kjs[l] * L + kp;
+#pragma unroll
+    for (int m = 0; m < E; ++m) z[l][m] = src[(t + TPL * m) * (L * L)];
+  }
+#pragma unroll
+  for (int l = 0; l < NL; ++l) line_fft<L, -1>(z[l], sre, sim, t, tw);
+  const int mjp = fold_sq(kjs[0], L) + fold_sq(kp, L);
   // |k|^2 and its square root for the lower-half positions (radius-independent)
   int mm[E / 2 + 1];
   SqrtPair sq[E / 2 + 1];
@@ -304,24 +318,24 @@ __global__ __launch_bounds__(TILE*(L / Plan<L>::E)) void k_filter_i(
 #pragma unroll 1
   for (int r = 0; r < nrad; ++r) {
     const int tl = opaque_lane(t);
-    float2 v[E];
+    float wv[E];
     if constexpr (WTAB) {
       const float* __restrict__ w = wlut + (long)r * mlut;
 #pragma unroll
-      for (int m = 0; m < E; ++m) {
-        const float wv = w[mjp + fold_sq(tl + TPL * m, L)];
-        v[m] = make_float2(z[m].x * wv, z[m].y * wv);
-      }
+      for (int m = 0; m < E; ++m) wv[m] = w[mjp + fold_sq(tl + TPL * m, L)];
     } else {
-      float wv[E];
       w_line_shared<L, E, TILE>(kappa[r], sq, mm, t, wv);  // dk R as hi + lo
-#pragma unroll
-      for (int m = 0; m < E; ++m) v[m] = make_float2(z[m].x * wv[m], z[m].y * wv[m]);
     }
-    line_fft<L, +1>(v, sre, sim, tl, tw);
-    float2* __restrict__ dst = u + (long)r * ustride + (long)kj * L + kp;
 #pragma unroll
-    for (int m = 0; m < E; ++m) dst[(tl + TPL * m) * (L * L)] = v[m];
+    for (int l = 0; l < NL; ++l) {
+      float2 v[E];
+#pragma unroll
+      for (int m = 0; m < E; ++m) v[m] = make_float2(z[l][m].x * wv[m], z[l][m].y * wv[m]);
+      line_fft<L, +1>(v, sre, sim, tl, tw);
+      float2* __restrict__ dst = u + (long)r * ustride + (long)kjs[l] * L + kp;
+#pragma unroll
+      for (int m = 0; m < E; ++m) dst[(tl + TPL * m) * (L * L)] = v[m];
+    }
   }
 }
 
@@ -698,16 +712,25 @@ void launch_cols(int n, int width, int axis, int sign, const float2* in, float2*
 
 void launch_filter_i(int n, const float2* spec, float2* u, int nrad, long ustride,
                      const float* wlut, long m_lut, const float2* kappa, const float2* tw,
-                     hipStream_t s, int tile, int wmode) {
+                     hipStream_t s, int tile, int wmode, bool jpair) {
 #define X(L, TL)                                                                          \
   if (n == L && tile == TL) {                                                             \
     constexpr int NT = TL * (L / Plan<L>::E);                                             \
+    if (jpair && TL >= 8 && wmode != 0) {                                                 \
+      /* mirror pairs (kj, L - kj), kj = 1 .. L/2 - 1, then the lines 0 and L/2 */        \
+      const dim3 g((L / 2 - 1) * (L / TL)), g0(2 * (L / TL));                             \
+      hipLaunchKernelGGL((k_filter_i<L, TL, false, true>), g, dim3(NT), 0, s, spec, u,    \
+                         nrad, ustride, wlut, m_lut, kappa, tw, 0);                       \
+      hipLaunchKernelGGL((k_filter_i<L, TL, false>), g0, dim3(NT), 0, s, spec, u, nrad,   \
+                         ustride, wlut, m_lut, kappa, tw, L / 2);                         \
+      return;                                                                             \
+    }                                                                                     \
     if (wmode == 0)                                                                       \
       hipLaunchKernelGGL((k_filter_i<L, TL, true>), dim3(L * (L / TL)), dim3(NT), 0, s, spec, u, \
-                         nrad, ustride, wlut, m_lut, kappa, tw);                          \
+                         nrad, ustride, wlut, m_lut, kappa, tw, 1);                       \
     else                                                                                  \
       hipLaunchKernelGGL((k_filter_i<L, TL, false>), dim3(L * (L / TL)), dim3(NT), 0, s, spec, u, \
-                         nrad, ustride, wlut, m_lut, kappa, tw);                          \
+                         nrad, ustride, wlut, m_lut, kappa, tw, 1);                       \
     return;                                                                               \
   }
 #define Y(L) X(L, 4) X(L, 8) X(L, 16)

@@ -81,6 +81,7 @@ struct sf21_hii_ctx {
   int cols_wide = 0;  // per-radius passes along j: bit 0 16 values per thread, bit 1 prefetching blocks
   bool rows_wide = false;  // 16 values per thread in the centre-test pass
   int filter_w = 1;  // 1: FP32 on the fly, 0: FP64 table per |k|^2
+  int filter_pair = 0;  // 1: the filter pass takes mirror pairs of i-lines
   float2* d_kappa = nullptr;  // dk R per radius as a float pair (hi, lo)
   long n3 = 0, nh = 0;
   double L = 0, dx = 0, dk = 0;
@@ -274,7 +275,7 @@ void enqueue_fused(sf21_hii_ctx* c, const float* a, const float* b, const float*
     const int kk = re - rb;
     launch(c, "filter_i", [&] {
       launch_filter_i(n, c->spec, c->ubuf, kk, n3, c->wlut + (long)rb * c->mlut, c->mlut,
-                      c->d_kappa + rb, c->tw, s, c->tile_filter, c->filter_w);
+                      c->d_kappa + rb, c->tw, s, c->tile_filter, c->filter_w, c->filter_pair);
     });
     launch(c, "cols_inv_j",
            [&] { launch_cols(n, n, 0, +1, c->ubuf, c->ubuf, kk, n3, c->tw, s, c->cols_tile, c->cols_wide); });
@@ -531,7 +532,7 @@ void enqueue_diff_backward(sf21_hii_ctx* c, const float* g, const float* emis, d
     const int kp = (kk + 1) / 2;
     launch(c, "filter_i", [&] {
       launch_filter_i(n, c->spec, c->ubuf, kk, n3, c->wlut + (long)rb * c->mlut, c->mlut,
-                      c->d_kappa + rb, c->tw, s, c->tile_filter, c->filter_w);
+                      c->d_kappa + rb, c->tw, s, c->tile_filter, c->filter_w, c->filter_pair);
     });
     launch(c, "cols_inv_j", [&] { launch_cols(n, n, 0, +1, c->ubuf, c->ubuf, kk, n3, c->tw, s, c->cols_tile, c->cols_wide); });
     launch(c, "rows_phi", [&] {
@@ -579,7 +580,7 @@ void enqueue_diff_backward(sf21_hii_ctx* c, const float* g, const float* emis, d
     if (!c->df_pairs) {  // recompute the batch's filtered pairs
       launch(c, "filter_i", [&] {
         launch_filter_i(n, c->spec, c->ubuf, kk, n3, c->wlut + (long)rb * c->mlut, c->mlut,
-                        c->d_kappa + rb, c->tw, s, c->tile_filter, c->filter_w);
+                        c->d_kappa + rb, c->tw, s, c->tile_filter, c->filter_w, c->filter_pair);
       });
       launch(c, "cols_inv_j", [&] { launch_cols(n, n, 0, +1, c->ubuf, c->ubuf, kk, n3, c->tw, s, c->cols_tile, c->cols_wide); });
     }
@@ -627,7 +628,7 @@ int diff_check(sf21_hii_ctx* c, const float* g, const float* emis, int nbins) {
 
 // ---- autotuner: choices of the fused path, persisted per (arch, n, radii) ----
 struct TuneChoice {
-  int tile = 16, K = 16, w = 1, ct = 16, cw = 0, rw = 0;
+  int tile = 16, K = 16, w = 1, ct = 16, cw = 0, rw = 0, fp = 0;
 };
 
 std::string tune_key(const sf21_hii_ctx* c) {
@@ -648,8 +649,8 @@ const char* tune_path(const char* given) {
   return (e && *e) ? e : nullptr;
 }
 
-// Cache lines: "<key> | tile=T K=K w=W ct=C cw=0..3 rw=0|1 | <ms> ms"; the
-// last match wins.
+// Cache lines: "<key> | tile=T K=K w=W ct=C cw=0..3 rw=0|1 fp=0|1 | <ms> ms"
+// (fp may be absent: 0); the last match wins.
 bool tune_lookup(const char* path, const std::string& key, TuneChoice* out) {
   FILE* f = path ? std::fopen(path, "r") : nullptr;
   if (!f) return false;
@@ -661,8 +662,8 @@ bool tune_lookup(const char* path, const std::string& key, TuneChoice* out) {
     std::string k(line, (size_t)(bar - line));
     while (!k.empty() && k.back() == ' ') k.pop_back();
     TuneChoice t;
-    if (k == key && std::sscanf(bar + 1, " tile=%d K=%d w=%d ct=%d cw=%d rw=%d", &t.tile, &t.K,
-                                &t.w, &t.ct, &t.cw, &t.rw) == 6) {
+    if (k == key && std::sscanf(bar + 1, " tile=%d K=%d w=%d ct=%d cw=%d rw=%d fp=%d", &t.tile,
+                                &t.K, &t.w, &t.ct, &t.cw, &t.rw, &t.fp) >= 6) {
       *out = t;
       found = true;
     }
@@ -674,8 +675,8 @@ bool tune_lookup(const char* path, const std::string& key, TuneChoice* out) {
 void tune_store(const char* path, const std::string& key, const TuneChoice& t, double ms) {
   FILE* f = path ? std::fopen(path, "a") : nullptr;
   if (!f) return;
-  std::fprintf(f, "%s | tile=%d K=%d w=%d ct=%d cw=%d rw=%d | %.3f ms\n", key.c_str(), t.tile,
-               t.K, t.w, t.ct, t.cw, t.rw, ms);
+  std::fprintf(f, "%s | tile=%d K=%d w=%d ct=%d cw=%d rw=%d fp=%d | %.3f ms\n", key.c_str(),
+               t.tile, t.K, t.w, t.ct, t.cw, t.rw, t.fp, ms);
   std::fclose(f);
 }
 
@@ -728,6 +729,7 @@ void apply_choice(sf21_hii_ctx* c, const TuneChoice& t) {
   c->cols_tile = t.ct;
   c->cols_wide = cols_variant(c->n, t.ct, t.cw);
   c->rows_wide = t.rw && wide_ok(c->n);
+  c->filter_pair = t.fp && t.tile >= 8 && t.w;
   drop_graphs(c);
 }
 
@@ -783,12 +785,15 @@ int sf21_hii_create(const sf21_hii_config* cfg, sf21_hii_ctx** out) {
   if (cfg->filter_tile == 4 || cfg->filter_tile == 8 || cfg->filter_tile == 16)
     c->tile_filter = cfg->filter_tile;
   c->filter_w = cfg->filter_w == 2 ? 0 : 1;
+  // untuned: mirror-paired filter blocks where the pairs still fill the GPU
+  c->filter_pair = c->n >= 256 && c->tile_filter >= 8 && c->filter_w;
   if (tuned) {
     c->tile_filter = tc.tile;
     c->filter_w = tc.w;
     c->cols_tile = tc.ct;
     c->cols_wide = cols_variant(c->n, tc.ct, tc.cw);
     c->rows_wide = tc.rw && wide_ok(c->n);
+    c->filter_pair = tc.fp && tc.tile >= 8 && tc.w;
   }
   if (c->K > (int)c->radii.size()) c->K = (int)c->radii.size();
 
@@ -1198,6 +1203,11 @@ int sf21_hii_autotune(sf21_hii_ctx* c, const float* a, const float* b, const flo
     consider(t);
   }
   {
+    TuneChoice t = best;
+    t.fp = 1 - t.fp;
+    consider(t);
+  }
+  {
     const TuneChoice base = best;
     const int cols_opts[6][2] = {{8, 0}, {16, 0}, {16, 2}, {16, 1}, {32, 1}, {32, 3}};
     for (auto& o : cols_opts) {
@@ -1227,8 +1237,9 @@ int sf21_hii_autotune(sf21_hii_ctx* c, const float* a, const float* b, const flo
 }
 
 int sf21_hii_tuning(const sf21_hii_ctx* c, int* filter_tile, int* radius_batch, int* filter_w,
-                    int* cols_tile, int* cols_wide, int* rows_wide) {
+                    int* cols_tile, int* cols_wide, int* rows_wide, int* filter_pair) {
   if (!c) return SF21_ERR_ARG;
+  if (filter_pair) *filter_pair = c->filter_pair;
   if (filter_tile) *filter_tile = c->tile_filter;
   if (radius_batch) *radius_batch = c->K;
   if (filter_w) *filter_w = c->filter_w ? 1 : 2;

@@ -26,7 +26,7 @@ def test_autotune_records_and_reloads(torch_cuda, tmp_path):
     t = ctx.tuning()
     print(f"\nautotune {n}^3, {len(radii)} radii: {ms:.3f} ms/slice with {t}", flush=True)
     assert ms > 0
-    assert t["filter_tile"] in (8, 16) and t["filter_w"] in (1, 2)
+    assert t["filter_tile"] in (8, 16) and t["filter_w"] in (1, 2) and t["filter_pair"] in (0, 1)
     assert (t["cols_tile"], t["cols_wide"]) in ((8, 0), (16, 0), (16, 2), (16, 1), (32, 1), (32, 3))
     assert t["radius_batch"] in (8, 12, 16, len(radii))
     lines = cache.read_text().strip().splitlines()
@@ -55,13 +55,14 @@ def test_autotune_records_and_reloads(torch_cuda, tmp_path):
             os.environ["SF21_TUNE_CACHE"] = old
 
 
-@pytest.mark.parametrize("cols,rows_wide", [((16, 1), 1), ((32, 1), 0), ((8, 0), 1), ((16, 2), 0),
-                                            ((32, 3), 1)])
-def test_wide_variants_are_bit_identical(torch_cuda, tmp_path, cols, rows_wide):
+@pytest.mark.parametrize("cols,rows_wide,fp", [((16, 1), 1, 0), ((32, 1), 0, 1), ((8, 0), 1, 0),
+                                               ((16, 2), 0, 1), ((32, 3), 1, 0)])
+def test_wide_variants_are_bit_identical(torch_cuda, tmp_path, cols, rows_wide, fp):
     """16 values per thread run the same butterflies on the same values as 8,
-    and the prefetching column blocks (cw bit 1) the same transforms on the
-    same tiles: the fused result is identical bit for bit (choices forced
-    through a cache line for this context's key)."""
+    the prefetching column blocks (cw bit 1) the same transforms on the same
+    tiles, and the filter blocks over mirror line pairs (fp) the same weights
+    (same |k|^2) and transforms: the fused result is identical bit for bit
+    (choices forced through a cache line for this context's key)."""
     torch = torch_cuda
     case = direct_case(128, zeta=1.0, seed=8200)
     n, side, radii = case["n"], case["side"], case["radii"]
@@ -73,13 +74,14 @@ def test_wide_variants_are_bit_identical(torch_cuda, tmp_path, cols, rows_wide):
     arch = torch.cuda.get_device_properties(0).gcnArchName.split(":")[0]
     cache = tmp_path / "tune.txt"
     cache.write_text(f"{arch} n={n} radii={len(radii)} | tile=16 K=16 w=1 ct={cols[0]} cw={cols[1]} "
-                     f"rw={rows_wide} | 0.0 ms\n")
+                     f"rw={rows_wide} fp={fp} | 0.0 ms\n")
     old = os.environ.get("SF21_TUNE_CACHE")
     os.environ["SF21_TUNE_CACHE"] = str(cache)
     try:
         ctx = sf.HiiContext(n, side, radii, sf.INPUT_DIRECT, sf.PATH_FUSED)
         t = ctx.tuning()
-        assert (t["cols_tile"], t["cols_wide"], t["rows_wide"]) == (cols[0], cols[1], rows_wide)
+        assert (t["cols_tile"], t["cols_wide"], t["rows_wide"], t["filter_pair"]) == \
+            (cols[0], cols[1], rows_wide, fp)
         x1, neu1 = ctx.run_slice(a, b)
         ctx.close()
     finally:
