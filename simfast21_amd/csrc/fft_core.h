@@ -110,6 +110,21 @@ struct Dft<5, SIGN> {
   }
 };
 
+// Streaming accesses of the per-radius fields: each element is read or
+// written once per pass and not again before the next pass has streamed the
+// whole (K n^3) buffer, so the accesses carry the non-temporal hint.
+typedef float sf21_v2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float2 ld_stream(const float2* p) {
+  const sf21_v2 v = __builtin_nontemporal_load(reinterpret_cast<const sf21_v2*>(p));
+  return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ void st_stream(float2* p, float2 a) {
+  sf21_v2 v;
+  v.x = a.x;
+  v.y = a.y;
+  __builtin_nontemporal_store(v, reinterpret_cast<sf21_v2*>(p));
+}
+
 // Compiler-level fence for the LDS exchanges.  __syncthreads() orders the
 // hardware, but the compiler was observed to sink a stage's LDS reads past
 // the following barrier into a later basic block (k_filter_i: the forward

@@ -171,7 +171,7 @@ __global__ __launch_bounds__(256) void k_fwd_rows(const float* __restrict__ a,
 // L x L x width array; TILE adjacent columns per block (128 B runs).
 // blockIdx.y selects one of several arrays spaced bstride apart.
 // ------------------------------------------------------------------------
-template <int L, int SIGN, int TILE, int E = Plan<L>::E>
+template <int L, int SIGN, int TILE, int E = Plan<L>::E, bool NT = false>
 __global__ __launch_bounds__(TILE*(L / E)) void k_cols(const float2* in, float2* out,
                                                        int width, int axis,
                                                        long bstride,
@@ -197,7 +197,10 @@ __global__ __launch_bounds__(TILE*(L / E)) void k_cols(const float2* in, float2*
   float2 v[E];
   if (ok) {
 #pragma unroll
-    for (int m = 0; m < E; ++m) v[m] = in[base + (long)(t + TPL * m) * stride];
+    for (int m = 0; m < E; ++m) {
+      const float2* p = in + base + (long)(t + TPL * m) * stride;
+      v[m] = NT ? ld_stream(p) : *p;
+    }
   } else {
 #pragma unroll
     for (int m = 0; m < E; ++m) v[m] = make_float2(0.f, 0.f);
@@ -205,7 +208,11 @@ __global__ __launch_bounds__(TILE*(L / E)) void k_cols(const float2* in, float2*
   line_fft_e<L, E, SIGN>(v, sre, sim, t, tw);
   if (ok) {
 #pragma unroll
-    for (int m = 0; m < E; ++m) out[base + (long)(t + TPL * m) * stride] = v[m];
+    for (int m = 0; m < E; ++m) {
+      float2* p = out + base + (long)(t + TPL * m) * stride;
+      if (NT) st_stream(p, v[m]);
+      else *p = v[m];
+    }
   }
 }
 
@@ -214,7 +221,7 @@ __global__ __launch_bounds__(TILE*(L / E)) void k_cols(const float2* in, float2*
 // and loading tile q + 1 into a second register set before transforming and
 // storing tile q, so that one resident block per CU keeps loads in flight
 // while it computes.  Tiles are disjoint, so the pass may run in place.
-template <int L, int SIGN, int TILE, int E>
+template <int L, int SIGN, int TILE, int E, bool NT>
 __global__ __launch_bounds__(TILE*(L / E)) void k_cols_pipe(const float2* in, float2* out,
                                                             int width, int axis, long bstride,
                                                             const float2* __restrict__ tw,
@@ -244,7 +251,9 @@ __global__ __launch_bounds__(TILE*(L / E)) void k_cols_pipe(const float2* in, fl
     const int t = (int)threadIdx.x / TILE;
 #pragma unroll
     for (int m = 0; m < E; ++m)
-      cur[m] = cb >= 0 ? in[cb + (long)(t + TPL * m) * stride] : make_float2(0.f, 0.f);
+      cur[m] = cb < 0 ? make_float2(0.f, 0.f)
+                      : (NT ? ld_stream(in + cb + (long)(t + TPL * m) * stride)
+                            : in[cb + (long)(t + TPL * m) * stride]);
   }
 #pragma unroll 1
   for (long q = q0; q < q1; ++q) {
@@ -254,12 +263,18 @@ __global__ __launch_bounds__(TILE*(L / E)) void k_cols_pipe(const float2* in, fl
       nb = origin(q + 1);
 #pragma unroll
       for (int m = 0; m < E; ++m)
-        nxt[m] = nb >= 0 ? in[nb + (long)(t + TPL * m) * stride] : make_float2(0.f, 0.f);
+        nxt[m] = nb < 0 ? make_float2(0.f, 0.f)
+                        : (NT ? ld_stream(in + nb + (long)(t + TPL * m) * stride)
+                              : in[nb + (long)(t + TPL * m) * stride]);
     }
     line_fft_e<L, E, SIGN>(cur, sre, sim, t, tw);
     if (cb >= 0) {
 #pragma unroll
-      for (int m = 0; m < E; ++m) out[cb + (long)(t + TPL * m) * stride] = cur[m];
+      for (int m = 0; m < E; ++m) {
+        float2* p = out + cb + (long)(t + TPL * m) * stride;
+        if (NT) st_stream(p, cur[m]);
+        else *p = cur[m];
+      }
     }
 #pragma unroll
     for (int m = 0; m < E; ++m) cur[m] = nxt[m];
@@ -334,7 +349,7 @@ __global__ __launch_bounds__(TILE*(L / Plan<L>::E)) void k_filter_i(
       line_fft<L, +1>(v, sre, sim, tl, tw);
       float2* __restrict__ dst = u + (long)r * ustride + (long)kjs[l] * L + kp;
 #pragma unroll
-      for (int m = 0; m < E; ++m) dst[(tl + TPL * m) * (L * L)] = v[m];
+      for (int m = 0; m < E; ++m) st_stream(dst + (tl + TPL * m) * (L * L), v[m]);
     }
   }
 }
@@ -372,7 +387,7 @@ __global__ __launch_bounds__(256) void k_inv_rows_crit(const float2* __restrict_
     const float2* __restrict__ src = vin + (long)r * vstride + row * L;
     float2 v[E];
 #pragma unroll
-    for (int m = 0; m < E; ++m) v[m] = src[tl + TPL * m];
+    for (int m = 0; m < E; ++m) v[m] = ld_stream(src + tl + TPL * m);
     line_fft_e<L, E, +1>(v, sre, sim, tl, tw);
     const int w = d2max[r];
 #pragma unroll
@@ -658,50 +673,56 @@ void launch_fwd_rows(int n, const float* a, const float* b, const float* c, Cell
 #undef X
 }
 
-template <int L, int TILE, int E = Plan<L>::E>
+template <int L, int TILE, int E, bool NT>
 static void cols_t(int width, int axis, int sign, const float2* in, float2* out, int nbatch,
                    long bstride, const float2* tw, hipStream_t s, bool pipe) {
-  constexpr int NT = TILE * (L / E);
+  constexpr int NTH = TILE * (L / E);
   if (pipe) {
     constexpr int kPer = 8;
     const long ntiles = (long)L * ((width + TILE - 1) / TILE) * nbatch;
     const dim3 g((unsigned)((ntiles + kPer - 1) / kPer));
     if (sign < 0)
-      hipLaunchKernelGGL((k_cols_pipe<L, -1, TILE, E>), g, dim3(NT), 0, s, in, out, width, axis,
+      hipLaunchKernelGGL((k_cols_pipe<L, -1, TILE, E, NT>), g, dim3(NTH), 0, s, in, out, width, axis,
                          bstride, tw, ntiles, kPer);
     else
-      hipLaunchKernelGGL((k_cols_pipe<L, +1, TILE, E>), g, dim3(NT), 0, s, in, out, width, axis,
+      hipLaunchKernelGGL((k_cols_pipe<L, +1, TILE, E, NT>), g, dim3(NTH), 0, s, in, out, width, axis,
                          bstride, tw, ntiles, kPer);
     return;
   }
   dim3 g(L * ((width + TILE - 1) / TILE), nbatch);
   if (sign < 0)
-    hipLaunchKernelGGL((k_cols<L, -1, TILE, E>), g, dim3(NT), 0, s, in, out, width, axis, bstride, tw);
+    hipLaunchKernelGGL((k_cols<L, -1, TILE, E, NT>), g, dim3(NTH), 0, s, in, out, width, axis, bstride, tw);
   else
-    hipLaunchKernelGGL((k_cols<L, +1, TILE, E>), g, dim3(NT), 0, s, in, out, width, axis, bstride, tw);
+    hipLaunchKernelGGL((k_cols<L, +1, TILE, E, NT>), g, dim3(NTH), 0, s, in, out, width, axis, bstride, tw);
 }
 
 // tile: columns per block, 16 (default) or 8; variant bit 0: 16 values per
 // thread (sizes 128-512; tile 16 or 32); bit 1: blocks walk 8 tiles with the
 // next tile's loads issued before the current tile's transform (tile 16, or
-// 32 with bit 0)
+// 32 with bit 0); bit 2: non-temporal loads and stores (whole streamed
+// buffers with width a multiple of the tile; the per-radius passes)
 void launch_cols(int n, int width, int axis, int sign, const float2* in, float2* out,
                  int nbatch, long bstride, const float2* tw, hipStream_t s, int tile, int variant) {
-  const bool wide = variant & 1, pipe = (variant & 2) != 0;
+  const bool wide = variant & 1, pipe = (variant & 2) != 0, nt = (variant & 4) != 0;
 #define X(L)                                                                              \
   if (n == L) {                                                                           \
+    constexpr int E0 = Plan<L>::E;                                                        \
     if (tile == 8)                                                                        \
-      cols_t<L, 8>(width, axis, sign, in, out, nbatch, bstride, tw, s, false);            \
+      cols_t<L, 8, E0, false>(width, axis, sign, in, out, nbatch, bstride, tw, s, false); \
+    else if (nt)                                                                          \
+      cols_t<L, 16, E0, true>(width, axis, sign, in, out, nbatch, bstride, tw, s, pipe);  \
     else                                                                                  \
-      cols_t<L, 16>(width, axis, sign, in, out, nbatch, bstride, tw, s, pipe);            \
+      cols_t<L, 16, E0, false>(width, axis, sign, in, out, nbatch, bstride, tw, s, pipe); \
     return;                                                                               \
   }
 #define XW(L)                                                                             \
   if (n == L && wide) {                                                                   \
-    if (tile == 32)                                                                       \
-      cols_t<L, 32, 16>(width, axis, sign, in, out, nbatch, bstride, tw, s, pipe);        \
+    if (tile == 32 && nt)                                                                 \
+      cols_t<L, 32, 16, true>(width, axis, sign, in, out, nbatch, bstride, tw, s, pipe);  \
+    else if (tile == 32)                                                                  \
+      cols_t<L, 32, 16, false>(width, axis, sign, in, out, nbatch, bstride, tw, s, pipe); \
     else                                                                                  \
-      cols_t<L, 16, 16>(width, axis, sign, in, out, nbatch, bstride, tw, s, false);       \
+      cols_t<L, 16, 16, false>(width, axis, sign, in, out, nbatch, bstride, tw, s, false); \
     return;                                                                               \
   }
   XW(128) XW(256) XW(512)

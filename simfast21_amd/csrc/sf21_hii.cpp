@@ -71,6 +71,11 @@ struct GraphEntry {
 
 }  // namespace
 
+// Column-pass variant bit for whole streamed complex buffers (width n): the
+// loads and stores carry the non-temporal hint.  The half-spectrum passes
+// (width n/2 + 1, tiles straddling cache lines) do not use it.
+constexpr int kColsStream = 4;
+
 struct sf21_hii_ctx {
   sf21_hii_config cfg{};
   std::vector<double> radii;
@@ -263,7 +268,7 @@ void enqueue_fused(sf21_hii_ctx* c, const float* a, const float* b, const float*
   hipStream_t s = c->stream;
   launch(c, "balance", [&] { launch_balance(n3, a, b, cc, k, c->bal_part, c->bal, s); });
   launch(c, "fwd_rows", [&] { launch_fwd_rows(n, a, b, cc, k, c->bal, c->spec, c->tw, s); });
-  launch(c, "cols_fwd_j", [&] { launch_cols(n, n, 0, -1, c->spec, c->spec, 1, 0, c->tw, s); });
+  launch(c, "cols_fwd_j", [&] { launch_cols(n, n, 0, -1, c->spec, c->spec, 1, 0, c->tw, s, 16, kColsStream); });
   launch(c, "fill_map", [&] { launch_fill_i32(c->wmap, -1, n3, s); });
   // radius batches in ascending radius (from the end of the descending
   // ladder): each batch's ball bounds exceed the earlier batches', which the
@@ -278,7 +283,7 @@ void enqueue_fused(sf21_hii_ctx* c, const float* a, const float* b, const float*
                       c->d_kappa + rb, c->tw, s, c->tile_filter, c->filter_w, c->filter_pair);
     });
     launch(c, "cols_inv_j",
-           [&] { launch_cols(n, n, 0, +1, c->ubuf, c->ubuf, kk, n3, c->tw, s, c->cols_tile, c->cols_wide); });
+           [&] { launch_cols(n, n, 0, +1, c->ubuf, c->ubuf, kk, n3, c->tw, s, c->cols_tile, c->cols_wide | kColsStream); });
     launch(c, "inv_rows_crit", [&] {
       launch_inv_rows_crit(n, c->ubuf, kk, n3, c->d_d2 + rb, c->bal, c->wmap, dbg, s, c->tw, c->rows_wide);
     });
@@ -479,8 +484,8 @@ int ensure_diff(sf21_hii_ctx* c) {
   for (size_t r = 0; r < c->radii.size(); ++r) {
     launch_ball_box(n, c->d2[r], c->df_c, s);
     launch_c2c_rows(n, -1, c->df_c, c->tw, s);
-    launch_cols(n, n, 0, -1, c->df_c, c->df_c, 1, 0, c->tw, s);
-    launch_cols(n, n, 1, -1, c->df_c, c->df_c, 1, 0, c->tw, s);
+    launch_cols(n, n, 0, -1, c->df_c, c->df_c, 1, 0, c->tw, s, 16, kColsStream);
+    launch_cols(n, n, 1, -1, c->df_c, c->df_c, 1, 0, c->tw, s, 16, kColsStream);
     launch_ball_fold(n, c->df_c, c->df_ball + r * h * h * h, s);
   }
   if (hipStreamSynchronize(s) != hipSuccess) return SF21_ERR_HIP;
@@ -524,7 +529,7 @@ void enqueue_diff_backward(sf21_hii_ctx* c, const float* g, const float* emis, d
   // forward spectrum Z of rec + i s ion, (p,j)-space
   launch(c, "balance", [&] { launch_balance(n3, c->df_rec, c->df_ion, nullptr, k, c->bal_part, c->bal, s); });
   launch(c, "fwd_rows", [&] { launch_fwd_rows(n, c->df_rec, c->df_ion, nullptr, k, c->bal, c->spec, c->tw, s); });
-  launch(c, "cols_fwd_j", [&] { launch_cols(n, n, 0, -1, c->spec, c->spec, 1, 0, c->tw, s); });
+  launch(c, "cols_fwd_j", [&] { launch_cols(n, n, 0, -1, c->spec, c->spec, 1, 0, c->tw, s, 16, kColsStream); });
   // sweep 1: C = sum_R B_R FFT(phi_R)
   launch(c, "diff_fill", [&] { launch_fill_c(c->df_c, n3, s); });
   for (int rb = 0; rb < nr; rb += c->K) {
@@ -534,17 +539,17 @@ void enqueue_diff_backward(sf21_hii_ctx* c, const float* g, const float* emis, d
       launch_filter_i(n, c->spec, c->ubuf, kk, n3, c->wlut + (long)rb * c->mlut, c->mlut,
                       c->d_kappa + rb, c->tw, s, c->tile_filter, c->filter_w, c->filter_pair);
     });
-    launch(c, "cols_inv_j", [&] { launch_cols(n, n, 0, +1, c->ubuf, c->ubuf, kk, n3, c->tw, s, c->cols_tile, c->cols_wide); });
+    launch(c, "cols_inv_j", [&] { launch_cols(n, n, 0, +1, c->ubuf, c->ubuf, kk, n3, c->tw, s, c->cols_tile, c->cols_wide | kColsStream); });
     launch(c, "rows_phi", [&] {
       launch_rows_phi(n, c->ubuf, kk, n3, c->bal, c->df_st, inv_tau, c->tw,
                       c->df_pairs ? c->df_pairs + (long)rb * n3 : nullptr, n3, s);
     });
-    launch(c, "cols_fwd_j", [&] { launch_cols(n, n, 0, -1, c->ubuf, c->ubuf, kp, n3, c->tw, s, c->cols_tile, c->cols_wide); });
+    launch(c, "cols_fwd_j", [&] { launch_cols(n, n, 0, -1, c->ubuf, c->ubuf, kp, n3, c->tw, s, c->cols_tile, c->cols_wide | kColsStream); });
     launch(c, "ball_acc_i", [&] {
       launch_ball_i(n, true, c->ubuf, n3, c->df_c, 0, kk, c->df_ball + (long)rb * h3, c->tw, s);
     });
   }
-  launch(c, "cols_inv_j", [&] { launch_cols(n, n, 0, +1, c->df_c, c->df_c, 1, 0, c->tw, s); });
+  launch(c, "cols_inv_j", [&] { launch_cols(n, n, 0, +1, c->df_c, c->df_c, 1, 0, c->tw, s, 16, kColsStream); });
   launch(c, "c2c_rows", [&] { launch_c2c_rows(n, +1, c->df_c, c->tw, s); });
   // relaxed state, its P(k) and the observables' adjoints
   launch(c, "relaxed_state", [&] {
@@ -564,7 +569,7 @@ void enqueue_diff_backward(sf21_hii_ctx* c, const float* g, const float* emis, d
                   c->df_c, c->df_qbar, c->df_dbar, s);
   });
   launch(c, "c2c_rows", [&] { launch_c2c_rows(n, -1, c->df_c, c->tw, s); });
-  launch(c, "cols_fwd_j", [&] { launch_cols(n, n, 0, -1, c->df_c, c->df_c, 1, 0, c->tw, s); });
+  launch(c, "cols_fwd_j", [&] { launch_cols(n, n, 0, -1, c->df_c, c->df_c, 1, 0, c->tw, s, 16, kColsStream); });
   // sweep 2: G = sum_R W_R FFT(seed_R)
   launch(c, "diff_fill", [&] {
     launch_fill_c(c->df_g, n3, s);
@@ -576,25 +581,25 @@ void enqueue_diff_backward(sf21_hii_ctx* c, const float* g, const float* emis, d
     launch(c, "ball_i", [&] {
       launch_ball_i(n, false, c->df_c, 0, c->df_v, n3, kk, c->df_ball + (long)rb * h3, c->tw, s);
     });
-    launch(c, "cols_inv_j", [&] { launch_cols(n, n, 0, +1, c->df_v, c->df_v, kp, n3, c->tw, s, c->cols_tile, c->cols_wide); });
+    launch(c, "cols_inv_j", [&] { launch_cols(n, n, 0, +1, c->df_v, c->df_v, kp, n3, c->tw, s, c->cols_tile, c->cols_wide | kColsStream); });
     if (!c->df_pairs) {  // recompute the batch's filtered pairs
       launch(c, "filter_i", [&] {
         launch_filter_i(n, c->spec, c->ubuf, kk, n3, c->wlut + (long)rb * c->mlut, c->mlut,
                         c->d_kappa + rb, c->tw, s, c->tile_filter, c->filter_w, c->filter_pair);
       });
-      launch(c, "cols_inv_j", [&] { launch_cols(n, n, 0, +1, c->ubuf, c->ubuf, kk, n3, c->tw, s, c->cols_tile, c->cols_wide); });
+      launch(c, "cols_inv_j", [&] { launch_cols(n, n, 0, +1, c->ubuf, c->ubuf, kk, n3, c->tw, s, c->cols_tile, c->cols_wide | kColsStream); });
     }
     launch(c, "rows_seed", [&] {
       launch_rows_seed(n, c->ubuf, c->df_v, kk, n3, n3, c->bal, c->df_st, inv_tau, c->df_rho, c->tw,
                        c->df_pairs ? c->df_pairs + (long)rb * n3 : nullptr, n3, s);
     });
-    launch(c, "cols_fwd_j", [&] { launch_cols(n, n, 0, -1, c->ubuf, c->ubuf, kk, n3, c->tw, s, c->cols_tile, c->cols_wide); });
+    launch(c, "cols_fwd_j", [&] { launch_cols(n, n, 0, -1, c->ubuf, c->ubuf, kk, n3, c->tw, s, c->cols_tile, c->cols_wide | kColsStream); });
     launch(c, "filter_acc_i", [&] {
       launch_filter_acc_i(n, c->ubuf, n3, kk, c->wlut + (long)rb * c->mlut, c->mlut, c->d_kappa + rb,
                           c->filter_w, c->df_g, c->tw, s);
     });
   }
-  launch(c, "cols_inv_j", [&] { launch_cols(n, n, 0, +1, c->df_g, c->df_g, 1, 0, c->tw, s); });
+  launch(c, "cols_inv_j", [&] { launch_cols(n, n, 0, +1, c->df_g, c->df_g, 1, 0, c->tw, s, 16, kColsStream); });
   launch(c, "c2c_rows", [&] { launch_c2c_rows(n, +1, c->df_g, c->tw, s); });
   launch(c, "grad_cells", [&] {
     launch_sum_rho(c->df_rho, rows_blocks(n), c->df_st, s);
@@ -713,6 +718,7 @@ int set_radius_batch(sf21_hii_ctx* c, int K) {
 }
 
 bool wide_ok(int n) { return n == 128 || n == 256 || n == 512; }
+
 
 // Column-pass variant that exists for this size and tile: 16 values per
 // thread only for wide_ok sizes (tile 16 or 32), prefetching blocks only
