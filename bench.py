@@ -97,14 +97,15 @@ def measured_traffic(kernel, n, n_radii, radius_batch):
     return rec.get("bytes_per_launch", {}).get(kernel)
 
 
-def bench_uneven(torch, sf, synth, n_boxes=64, steps=3, seed=4000):
+def bench_uneven(torch, sf, synth, n_boxes=64, steps=3, seed=4000, tune=True):
     """Parameter-sweep batch: n_boxes boxes with N drawn uniformly from
     {64, 96, 128, 192, 256} at a fixed cell size dx = 300/256 Mpc (sides
     75-300 Mpc), P(k) slope ~ U[-2.8, -2.2], sigma 1.2, 4000 (N/128)^3
     sources, efficiency zeta ~ 1.5 lognormal(0, 0.5); radius ladder per size
     from R_max = side/4 down by 0.9 to the cell size.  All boxes in one
-    hipGraph launch (sf21_hii_run_multi_dev); per-size throughput from each
-    box run alone."""
+    hipGraph launch (sf21_hii_run_multi_dev: boxes of one size in order,
+    sizes as parallel branches); launch choices autotuned per size on its
+    first box (not timed); per-size throughput from each box run alone."""
     rng = np.random.default_rng(seed)
     gen = torch.Generator(device="cuda")
     gen.manual_seed(seed)
@@ -125,6 +126,10 @@ def bench_uneven(torch, sf, synth, n_boxes=64, steps=3, seed=4000):
         a.append((e ** 2).float().reshape(-1).contiguous())
         del g, e
     boxes = [ctxs[n] for n in sizes]
+    if tune:
+        for n, c in ctxs.items():
+            k = sizes.index(n)
+            c.autotune(a[k], b[k])
     outs = [torch.empty_like(x) for x in a]
     sf.run_multi(boxes, a, b, out_list=outs)  # capture
     torch.cuda.synchronize()
@@ -414,7 +419,7 @@ def main():
     uneven = None
     if not args.no_uneven:
         try:
-            uneven = bench_uneven(torch, sf, synth)
+            uneven = bench_uneven(torch, sf, synth, tune=not args.no_tune)
         finally:
             torch.cuda.empty_cache()
         print(f"[bench] uneven batch: {uneven['ms_per_launch']:.1f} ms per launch, "

@@ -1041,22 +1041,26 @@ int sf21_hii_run_multi_dev(int nbox, sf21_hii_ctx* const* ctx, const float* cons
   if (hipSetDevice(c0->cfg.device) != hipSuccess) return SF21_ERR_HIP;
   (void)hipGetLastError();
   hipStream_t s = c0->stream;
-  // every context launches on the first one's stream for this call
   std::vector<hipStream_t> saved;
-  for (auto* c : uniq) {
-    saved.push_back(c->stream);
-    c->stream = s;
-  }
+  for (auto* c : uniq) saved.push_back(c->stream);
   bool prof = false;
   for (auto* c : uniq) prof = prof || c->prof;
   int rc = SF21_OK;
-  auto enqueue_all = [&] {
+  // boxes of one context run in box order on that context's stream (they
+  // share its buffers); different contexts share nothing and run as
+  // parallel branches of the graph
+  auto enqueue_ctx = [&](sf21_hii_ctx* c) {
+    for (int i = 0; i < nbox; ++i)
+      if (ctx[i] == c)
+        enqueue_fused(c, a[i], b[i], cc ? cc[i] : nullptr, z[i], xhii[i], c->neutral_dev + slot[i],
+                      -1, nullptr);
+  };
+  if (prof) {
+    // profiled calls: every context on the first one's stream, in box order
+    for (auto* c : uniq) c->stream = s;
     for (int i = 0; i < nbox; ++i)
       enqueue_fused(ctx[i], a[i], b[i], cc ? cc[i] : nullptr, z[i], xhii[i],
                     ctx[i]->neutral_dev + slot[i], -1, nullptr);
-  };
-  if (prof) {
-    enqueue_all();
   } else {
     // one hipGraph for the whole set, cached on the first context by
     // (contexts, pointers, redshifts)
@@ -1074,14 +1078,30 @@ int sf21_hii_run_multi_dev(int nbox, sf21_hii_ctx* const* ctx, const float* cons
       if (g.ptrs == key && g.zs == zs) ge = &g;
     if (!ge) {
       hipGraph_t graph = nullptr;
-      if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+      std::vector<hipEvent_t> ev(uniq.size() + 1, nullptr);
+      for (auto& e : ev)
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) rc = SF21_ERR_HIP;
+      if (!rc && hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
         rc = SF21_ERR_HIP;
-      } else {
+      } else if (!rc) {
         for (auto* c : uniq) c->capturing = true;
-        enqueue_all();
+        // fork: every other context's stream joins the capture after s's
+        // fork point, runs its boxes, and s waits for it before the end
+        hipError_t e = hipEventRecord(ev[0], s);
+        for (size_t k = 1; k < uniq.size() && e == hipSuccess; ++k)
+          e = hipStreamWaitEvent(uniq[k]->stream, ev[0], 0);
+        if (e == hipSuccess) {
+          for (auto* c : uniq) enqueue_ctx(c);
+          for (size_t k = 1; k < uniq.size() && e == hipSuccess; ++k) {
+            e = hipEventRecord(ev[k], uniq[k]->stream);
+            if (e == hipSuccess) e = hipStreamWaitEvent(s, ev[k], 0);
+          }
+        }
         for (auto* c : uniq) c->capturing = false;
-        if (hipStreamEndCapture(s, &graph) != hipSuccess) rc = SF21_ERR_HIP;
+        if (hipStreamEndCapture(s, &graph) != hipSuccess || e != hipSuccess) rc = SF21_ERR_HIP;
       }
+      for (auto& e : ev)
+        if (e) (void)hipEventDestroy(e);
       if (!rc) {
         GraphEntry g;
         g.ptrs = key;

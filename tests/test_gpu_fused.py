@@ -92,7 +92,17 @@ def check_vs_oracle(torch, case, filter_w=0, dilate=orc.dilate_brute, label=""):
                 ok = near_threshold(rr[cells], ii[cells], err_rec, err_ion)
                 assert ok[flip].all(), (k, cells[flip & ~ok][:5], (ii[cells] / rr[cells])[flip & ~ok][:5])
                 explained |= flip
-                print(f"  radius {k} R={radii[k]:.3f}: {int(flip.sum())} flipped of {cells.size}, "
+                # which threshold each flip sits at: the FP64 ratio ion/rec
+                # within 1e-4 of 1; ion within its FP32 error of 0 (the
+                # criterion's ion > 0 test); else within the absolute FP32
+                # field errors of both fields (small fields)
+                dev = np.abs(ii[cells] - rr[cells]) / np.maximum(np.abs(rr[cells]), np.abs(ii[cells]))
+                at_ratio = flip & (dev <= 1e-4)
+                at_sign = flip & ~at_ratio & (np.abs(ii[cells]) <= err_ion)
+                at_abs = flip & ~at_ratio & ~at_sign
+                print(f"  radius {k} R={radii[k]:.3f}: {int(flip.sum())} flipped of {cells.size} "
+                      f"(ratio within 1e-4: {int(at_ratio.sum())}, ion within FP32 error of 0: "
+                      f"{int(at_sign.sum())}, within absolute field error: {int(at_abs.sum())}), "
                       f"rel err rec {rel_rec:.2e} ion {rel_ion:.2e}", flush=True)
             assert explained.all(), cells[~explained][:5]
         ctx.close()
