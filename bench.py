@@ -169,7 +169,9 @@ def bench_diff(torch, sf, n, side, radii, n_src, taus=(0.05, 0.02, 0.01), reps=3
     """Differentiable mode at one redshift: Gaussian g (sigma 1.2, P ~ k^-2.5),
     emissivity pattern from n_src sources ~ (1+delta)^2, zeta set for
     <x> ~ 0.5.  Times the exact forward (fields, bubble finder, T, P(k)) and
-    the relaxed backward (recomputing every per-radius field) per tau."""
+    the relaxed backward per tau: recomputing every per-radius field in both
+    sweeps, keeping the first sweep's for the second, and with the forward
+    keeping them for the backward."""
     gen = torch.Generator(device="cuda")
     gen.manual_seed(seed)
     g = gaussian_field_dev(torch, gen, n).reshape(-1).contiguous()
@@ -233,6 +235,40 @@ def bench_diff(torch, sf, n, side, radii, n_src, taus=(0.05, 0.02, 0.01), reps=3
         print(f"[bench] diff backward with stored fields {bwd_stored}", file=sys.stderr, flush=True)
     except sf.Sf21Error as e:  # does not fit
         bwd_stored = {"error": str(e)}
+    # forward keeps the fields (mode 3): training-loop pairs of forward and
+    # backward on the same inputs; the forward writes the pairs as it finds
+    # the centres, the backward's first sweep reads them
+    kept = None
+    try:
+        ctx.diff_store_fields(3)
+        ctx.diff_forward(g, emis, zeta, 20, x_out=x)
+        ctx.diff_backward(g, emis, zeta, taus[0], 1.0, cp, 20, grad_out=grad)
+        torch.cuda.synchronize()
+        kf, kb, used = 0.0, {}, True
+        for tau in taus:
+            tf = tb = 0.0
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                ctx.diff_forward(g, emis, zeta, 20, x_out=x)
+                torch.cuda.synchronize()
+                t1 = time.perf_counter()
+                ctx.diff_backward(g, emis, zeta, tau, 1.0, cp, 20, grad_out=grad)
+                torch.cuda.synchronize()
+                t2 = time.perf_counter()
+                used = used and ctx.diff_last_kept
+                tf += t1 - t0
+                tb += t2 - t1
+            kf += tf
+            kb[str(tau)] = round(tb * 1e3 / reps, 2)
+        kf_ms = kf * 1e3 / (reps * len(taus))
+        kept = {"forward_ms": round(kf_ms, 2), "backward_ms": kb, "backward_read_kept_fields": used,
+                "grad_over_forward": {k: round(v / kf_ms, 3) for k, v in kb.items()},
+                "forward_plus_backward_ms": {k: round(kf_ms + v, 2) for k, v in kb.items()},
+                "forward_plus_backward_ms_recompute": {k: round(fwd_ms + v, 2) for k, v in bwd.items()}}
+        print(f"[bench] diff forward keeping fields {kf_ms:.1f} ms, backward {kb}, "
+              f"grad/forward {kept['grad_over_forward']}", file=sys.stderr, flush=True)
+    except sf.Sf21Error as e:  # does not fit
+        kept = {"error": str(e)}
     ctx.diff_store_fields(0)
     ctx_tuning = ctx.tuning()
     # per-kernel split of one backward (recompute mode)
@@ -255,6 +291,7 @@ def bench_diff(torch, sf, n, side, radii, n_src, taus=(0.05, 0.02, 0.01), reps=3
         "tuning": ctx_tuning,
         "grad_over_forward_fields_stored": ({k: round(v / fwd_ms, 3) for k, v in bwd_stored.items()}
                                             if bwd_stored and "error" not in bwd_stored else None),
+        "forward_keeps_fields": kept,
         "backward_kernels_ms": {k: [c, round(t, 3)] for k, (c, t) in kst.items()},
     }
 

@@ -181,16 +181,25 @@ int sf21_nldensity_dev(sf21_hii_ctx *ctx, const sf21_params *p, double z, const 
  * the union of balls becomes x~ = q + (1 - q)(1 - exp(-sum_R B_R * phi_R)),
  * B_R the exact ball, q the quick fill) with respect to g (grad_g, device
  * n^3) and zeta (*grad_zeta), plus the relaxed <x~> and P_b (host,
- * nullable).  Per-radius fields are recomputed in the backward (never
- * stored by the forward; see sf21_hii_diff_store_fields); tau -> 0 gives
- * back the exact forward.  Sums in FP64. */
-/* Field storage in the backward: the first sweep recomputes every radius'
+ * nullable).  Per-radius fields are recomputed in the backward unless
+ * sf21_hii_diff_store_fields asks otherwise; tau -> 0 gives back the exact
+ * forward.  Sums in FP64. */
+/* Field storage: the backward's first sweep recomputes every radius'
  * filtered pair from the forward spectrum; mode 1 keeps them (n_R n^3
  * complex floats, 47 GB at 512^3 with 47 radii) for the second sweep, mode 0
  * recomputes them there as well, mode 2 (default) keeps them when they take
- * at most half of the free device memory.  The gradient is identical;
- * SF21_ERR_ALLOC if mode 1 does not fit. */
+ * at most half of the free device memory.  Mode 3: the exact forward keeps
+ * them (its centre-test pass writes them, +8 B per voxel and radius) and the
+ * next backward reads them instead of recomputing its first sweep -- when
+ * its (g, emis, zeta) carry the same FP64 fingerprints as the forward's
+ * (weighted sums of g and emis; an input changed in place between the two
+ * calls is detected unless the change preserves both sums); otherwise it
+ * recomputes as in mode 1.  The gradient is identical in every mode;
+ * SF21_ERR_ALLOC if mode 1 or 3 does not fit. */
 int sf21_hii_diff_store_fields(sf21_hii_ctx *ctx, int mode);
+/* 1 if the last diff_backward read fields kept by the forward (mode 3), 0
+ * otherwise. */
+int sf21_hii_diff_last_kept(const sf21_hii_ctx *ctx);
 int sf21_hii_diff_forward_dev(sf21_hii_ctx *ctx, const float *g, const float *emis, double zeta,
                               int nbins, float *x_out, double *mean_x, double *pk);
 int sf21_hii_diff_backward_dev(sf21_hii_ctx *ctx, const float *g, const float *emis,

@@ -128,6 +128,7 @@ def _lib():
     L.sf21_hii_power_spectrum_dev.argtypes = [vp, vp, i, vp, vp, vp]
     L.sf21_hii_halo_emissivity_dev.argtypes = [vp, vp, ctypes.c_long, d, vp]
     L.sf21_hii_diff_store_fields.argtypes = [vp, i]
+    L.sf21_hii_diff_last_kept.argtypes = [vp]
     L.sf21_sfrd_dev.argtypes = [vp, vp, ctypes.c_long, d, vp, dp]
     L.sf21_nldensity_dev.argtypes = [vp, ctypes.POINTER(Params), d, vp, vp, vp, vp, vp, vp]
     L.sf21_t21_dev.argtypes = [vp, ctypes.POINTER(Params), d, vp, vp, vp, vp, vp, vp, dp, dp]
@@ -312,8 +313,14 @@ class HiiContext:
     def diff_store_fields(self, mode: int = 1):
         """Backward field storage: 1 keep the first sweep's per-radius pairs
         (n_R n^3 complex floats) for the second, 0 recompute them, 2 keep them
-        if they fit in half the free device memory (the default)."""
+        if they fit in half the free device memory (the default), 3 the exact
+        forward keeps them for the next backward on the same inputs."""
         _check(_lib().sf21_hii_diff_store_fields(self._h, int(mode)), "sf21_hii_diff_store_fields")
+
+    @property
+    def diff_last_kept(self) -> bool:
+        """Whether the last diff_backward read fields kept by the forward."""
+        return bool(_lib().sf21_hii_diff_last_kept(self._h))
 
     def diff_forward(self, g, emis, zeta: float, nbins: int = 20, x_out=None):
         """Exact forward of the differentiable model on device boxes g, emis:

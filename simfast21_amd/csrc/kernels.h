@@ -39,7 +39,8 @@ void launch_filter_i(int n, const float2* spec, float2* u, int nrad, long ustrid
                      hipStream_t s, int tile, int wmode, bool jpair = false);
 void launch_inv_rows_crit(int n, const float2* v, int nrad, long vstride, const int* d2max,
                           const float* bal, int* wmap, float2* dbg, hipStream_t s,
-                          const float2* tw, bool wide = false);
+                          const float2* tw, bool wide = false,
+                          float2* keep = nullptr);
 void launch_fill_i32(int* p, int value, long count, hipStream_t s);
 void launch_dt_rows(const DtArgs& a, const int* in, int* out, hipStream_t s);
 void launch_dt_cols(const DtArgs& a, const int* in, int* out, hipStream_t s);
@@ -99,8 +100,14 @@ void launch_ball_box(int n, int d2, float2* out, hipStream_t s);
 void launch_ball_fold(int n, const float2* spec, float* table, hipStream_t s);
 // pairs (nullable): per-radius filtered pairs, stored by rows_phi and read
 // by rows_seed instead of a recomputation (field-storage mode)
+// kept: pairs holds the batch's real-space pairs (kept by the forward pass);
+// no inverse transform of u and no store into pairs
 void launch_rows_phi(int n, float2* u, int nrad, long ustride, const float* bal, const double* st,
-                     float inv_tau, const float2* tw, float2* pairs, long pstride, hipStream_t s);
+                     float inv_tau, const float2* tw, float2* pairs, long pstride, hipStream_t s,
+                     bool kept = false);
+// FP64 fingerprints of g and emis into out[0], out[1] (part: eb(n3) pairs)
+void launch_fingerprint(long n3, const float* g, const float* emis, double* part, double* out,
+                        hipStream_t s);
 void launch_rows_seed(int n, float2* u, const float2* v, int nrad, long ustride, long vstride,
                       const float* bal, const double* st, float inv_tau, double* rho_acc,
                       const float2* tw, const float2* pairs, long pstride, hipStream_t s);

@@ -86,6 +86,28 @@ __global__ __launch_bounds__(256) void k_sum_exp(long n3, const float* __restric
   }
 }
 
+// Fingerprint of the inputs (g, E): FP64 sums of g_i w_i and E_i w_i with
+// w_i = 1 + (i mod 8191) 2^-13, over fixed per-block partials (deterministic
+// for a given launch shape).  Used to tell whether the fields kept by a
+// forward pass belong to the inputs of the following backward pass.
+__global__ __launch_bounds__(256) void k_fingerprint(long n3, const float* __restrict__ g,
+                                                     const float* __restrict__ emis,
+                                                     double* __restrict__ part) {
+  __shared__ double r1[256], r2[256];
+  double a = 0.0, b = 0.0;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n3;
+       i += (long)gridDim.x * blockDim.x) {
+    const double w = 1.0 + (double)(i % 8191) * (1.0 / 8192.0);
+    a += (double)g[i] * w;
+    b += (double)emis[i] * w;
+  }
+  block_reduce2(a, b, r1, r2);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = r1[0];
+    part[2 * blockIdx.x + 1] = r2[0];
+  }
+}
+
 // Delta = exp(g) N / sum exp(g), rec = Delta^2, ion = zeta E; sum(rec) partials.
 __global__ __launch_bounds__(256) void k_make_fields(long n3, const float* __restrict__ g,
                                                      const float* __restrict__ emis, float zf,
@@ -183,7 +205,10 @@ __device__ __forceinline__ float rho_u_of(const double* st) {
 // and transformed forward along p, written in place to slot r/2 (slot r/2's
 // row of this block was read at an earlier or the same iteration).
 // ---------------------------------------------------------------------------
-template <int L>
+// KEPT: the batch's real-space pairs come from pairs (kept by the exact
+// forward pass) instead of an inverse transform of u; u only receives the
+// phi pairs.
+template <int L, bool KEPT = false>
 __global__ __launch_bounds__(256) void k_rows_phi(float2* u, int nrad, long ustride,
                                                   const float* __restrict__ bal,
                                                   const double* __restrict__ st, float inv_tau,
@@ -204,11 +229,17 @@ __global__ __launch_bounds__(256) void k_rows_phi(float2* u, int nrad, long ustr
   for (int r = 0; r < nrad; ++r) {
     const int tl = opaque_lane(t);
     float2 v[E];
-    const float2* src = u + (long)r * ustride + row * L;
+    if constexpr (KEPT) {
+      const float2* src = pairs + (long)r * pstride + row * L;
 #pragma unroll
-    for (int m = 0; m < E; ++m) v[m] = src[tl + TPL * m];
-    line_fft<L, +1>(v, sre, sim, tl, tw);
-    if (pairs) {
+      for (int m = 0; m < E; ++m) v[m] = ld_stream(src + tl + TPL * m);
+    } else {
+      const float2* src = u + (long)r * ustride + row * L;
+#pragma unroll
+      for (int m = 0; m < E; ++m) v[m] = src[tl + TPL * m];
+      line_fft<L, +1>(v, sre, sim, tl, tw);
+    }
+    if (pairs && !KEPT) {
       // stored for the second sweep (field-storage mode)
       float2* dst = pairs + (long)r * pstride + row * L;
 #pragma unroll
@@ -647,10 +678,16 @@ void launch_ball_fold(int n, const float2* spec, float* table, hipStream_t s) {
 
 template <int L>
 static void rows_phi_t(int n, float2* u, int nrad, long ustride, const float* bal, const double* st,
-                       float inv_tau, const float2* tw, float2* pairs, long pstride, hipStream_t s) {
+                       float inv_tau, const float2* tw, float2* pairs, long pstride, bool kept,
+                       hipStream_t s) {
   constexpr int LPB = 256 / (L / Plan<L>::E);
-  hipLaunchKernelGGL(k_rows_phi<L>, dim3((unsigned)((long)n * n / LPB)), dim3(256), 0, s, u, nrad,
-                     ustride, bal, st, inv_tau, tw, pairs, pstride);
+  const dim3 g((unsigned)((long)n * n / LPB));
+  if (kept)
+    hipLaunchKernelGGL((k_rows_phi<L, true>), g, dim3(256), 0, s, u, nrad, ustride, bal, st,
+                       inv_tau, tw, pairs, pstride);
+  else
+    hipLaunchKernelGGL((k_rows_phi<L, false>), g, dim3(256), 0, s, u, nrad, ustride, bal, st,
+                       inv_tau, tw, pairs, pstride);
 }
 template <int L>
 static void rows_seed_t(int n, float2* u, const float2* v, int nrad, long ustride, long vstride,
@@ -705,10 +742,16 @@ int rows_blocks(int n) {
   }
   return 0;
 }
+void launch_fingerprint(long n3, const float* g, const float* emis, double* part, double* out,
+                        hipStream_t s) {
+  hipLaunchKernelGGL(k_fingerprint, dim3(eb(n3)), dim3(256), 0, s, n3, g, emis, part);
+  hipLaunchKernelGGL(k_final2, dim3(1), dim3(256), 0, s, part, (int)eb(n3), out, 0, 1);
+}
 void launch_rows_phi(int n, float2* u, int nrad, long ustride, const float* bal, const double* st,
-                     float inv_tau, const float2* tw, float2* pairs, long pstride, hipStream_t s) {
+                     float inv_tau, const float2* tw, float2* pairs, long pstride, hipStream_t s,
+                     bool kept) {
   switch (n) {
-#define C(L) case L: rows_phi_t<L>(n, u, nrad, ustride, bal, st, inv_tau, tw, pairs, pstride, s); break;
+#define C(L) case L: rows_phi_t<L>(n, u, nrad, ustride, bal, st, inv_tau, tw, pairs, pstride, kept, s); break;
     SF21_DIFF_SIZES(C)
 #undef C
   }

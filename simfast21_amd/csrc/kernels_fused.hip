@@ -361,7 +361,8 @@ __global__ __launch_bounds__(TILE*(L / Plan<L>::E)) void k_filter_i(
 // ascending radius, so a batch's bounds exceed every earlier batch's and a
 // plain store (only where some centre was found) leaves in wmap the bound of
 // the largest radius -- no atomics and no reads of the map.  dbg (nullable)
-// receives the unscaled filtered pairs.
+// receives the unscaled filtered pairs, keep (nullable) the pairs as
+// computed (N^3 rec_R, s N^3 ion_R), for the differentiable mode's backward.
 // ------------------------------------------------------------------------
 template <int L, int E = Plan<L>::E>
 __global__ __launch_bounds__(256) void k_inv_rows_crit(const float2* __restrict__ vin,
@@ -370,6 +371,7 @@ __global__ __launch_bounds__(256) void k_inv_rows_crit(const float2* __restrict_
                                                        const float* __restrict__ bal,
                                                        int* __restrict__ wmap,
                                                        float2* __restrict__ dbg,
+                                                       float2* __restrict__ keep,
                                                        const float2* __restrict__ tw) {
   constexpr int TPL = L / E, LPB = 256 / TPL;
   __shared__ float lds[LPB * 2 * LdsLine<L>::kFloats];
@@ -395,6 +397,7 @@ __global__ __launch_bounds__(256) void k_inv_rows_crit(const float2* __restrict_
       // Re = N^3 rec_R, Im = s N^3 ion_R: centre test on (s rec_R, s ion_R)
       if (is_centre(v[m].x * sc, v[m].y)) wbest[m] = max(wbest[m], w);
       if (dbg) dbg[(long)r * vstride + row * L + tl + TPL * m] = make_float2(v[m].x, v[m].y / sc);
+      if (keep) st_stream(keep + (long)r * vstride + row * L + tl + TPL * m, v[m]);
     }
   }
 #pragma unroll
@@ -762,19 +765,19 @@ void launch_filter_i(int n, const float2* spec, float2* u, int nrad, long ustrid
 
 void launch_inv_rows_crit(int n, const float2* v, int nrad, long vstride, const int* d2max,
                           const float* bal, int* wmap, float2* dbg, hipStream_t s,
-                          const float2* tw, bool wide) {
+                          const float2* tw, bool wide, float2* keep) {
 #define XW(L)                                                                             \
   if (n == L && wide) {                                                                   \
     constexpr int LPB = 256 / (L / 16);                                                   \
     hipLaunchKernelGGL((k_inv_rows_crit<L, 16>), dim3((L * L) / LPB), dim3(256), 0, s, v, nrad, \
-                       vstride, d2max, bal, wmap, dbg, tw);                               \
+                       vstride, d2max, bal, wmap, dbg, keep, tw);                               \
     return;                                                                               \
   }
 #define X(L)                                                                              \
   if (n == L) {                                                                           \
     constexpr int LPB = 256 / (L / Plan<L>::E);                                           \
     hipLaunchKernelGGL(k_inv_rows_crit<L>, dim3((L * L) / LPB), dim3(256), 0, s, v, nrad, vstride, \
-                       d2max, bal, wmap, dbg, tw);                                        \
+                       d2max, bal, wmap, dbg, keep, tw);                                        \
     return;                                                                               \
   }
   XW(128) XW(256) XW(512)

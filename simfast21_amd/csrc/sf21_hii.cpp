@@ -130,7 +130,13 @@ struct sf21_hii_ctx {
   double* df_coef = nullptr;
   double* df_host = nullptr;  // pinned staging
   float2* df_pairs = nullptr; // field-storage mode: n_R x n^3 filtered pairs
-  int df_store = 2;           // 0 recompute, 1 store, 2 store if it fits in half the free HBM
+  int df_store = 2;           // 0 recompute, 1 store, 2 store if it fits in half the free HBM,
+                              // 3 the exact forward keeps them for the backward
+  double* df_fp = nullptr;    // input fingerprints (device): [0..1] kept, [2..3] current
+  bool df_kept = false;       // df_pairs hold the fields of the inputs fingerprinted below
+  double df_kept_fp[2] = {0.0, 0.0};
+  double df_kept_zeta = 0.0;
+  bool df_last_kept = false;  // the last backward read forward-kept fields
   bool df_ready = false;
   // 21-cm step
   float* t21_dv = nullptr;      // velocity gradient (unnormalised C2R output)
@@ -260,8 +266,10 @@ int check_hip() {
 }
 
 // ---- fused exact path: enqueue one slice -------------------------------
+// keep (nullable): the filtered pairs of every radius, radius r at keep + r n^3
 void enqueue_fused(sf21_hii_ctx* c, const float* a, const float* b, const float* cc, double z,
-                   float* xhii, double* neutral_slot, int only_radius, float2* dbg) {
+                   float* xhii, double* neutral_slot, int only_radius, float2* dbg,
+                   float2* keep = nullptr) {
   const int n = c->n;
   const long n3 = c->n3;
   const CellConsts k = cell_consts(c, z);
@@ -285,7 +293,8 @@ void enqueue_fused(sf21_hii_ctx* c, const float* a, const float* b, const float*
     launch(c, "cols_inv_j",
            [&] { launch_cols(n, n, 0, +1, c->ubuf, c->ubuf, kk, n3, c->tw, s, c->cols_tile, c->cols_wide | kColsStream); });
     launch(c, "inv_rows_crit", [&] {
-      launch_inv_rows_crit(n, c->ubuf, kk, n3, c->d_d2 + rb, c->bal, c->wmap, dbg, s, c->tw, c->rows_wide);
+      launch_inv_rows_crit(n, c->ubuf, kk, n3, c->d_d2 + rb, c->bal, c->wmap, dbg, s, c->tw, c->rows_wide,
+                           keep ? keep + (long)rb * n3 : nullptr);
     });
   }
   if (only_radius >= 0) return;
@@ -392,6 +401,7 @@ void free_all(sf21_hii_ctx* c) {
   dev_free(c->df_dbar); dev_free(c->df_dd); dev_free(c->df_v); dev_free(c->df_c);
   dev_free(c->df_g); dev_free(c->df_half); dev_free(c->df_ball); dev_free(c->df_st);
   dev_free(c->df_part); dev_free(c->df_rho); dev_free(c->df_gp); dev_free(c->df_coef);
+  dev_free(c->df_fp);
   dev_free(c->df_pairs);
   if (c->df_host) (void)hipHostFree(c->df_host);
   c->df_host = nullptr;
@@ -472,7 +482,7 @@ int ensure_diff(sf21_hii_ctx* c) {
       (rc = dev_alloc(&c->df_st, (size_t)DS_COUNT)) ||
       (rc = dev_alloc(&c->df_part, (size_t)2 * diff_partials_count(c->n3))) ||
       (rc = dev_alloc(&c->df_rho, (size_t)rows_blocks(c->n))) || (rc = dev_alloc(&c->df_gp, 64)) ||
-      (rc = dev_alloc(&c->df_coef, 64)))
+      (rc = dev_alloc(&c->df_coef, 64)) || (rc = dev_alloc(&c->df_fp, 4)))
     return rc;
   if (hipHostMalloc(reinterpret_cast<void**>(&c->df_host), sizeof(double) * 256) != hipSuccess) {
     c->df_host = nullptr;
@@ -507,17 +517,21 @@ void enqueue_diff_fields(sf21_hii_ctx* c, const float* g, const float* emis, dou
 // Exact forward: x (bubble finder on rec = Delta^2, ion = zeta emis, direct
 // inputs), T = (1 - x) Delta, sum(x), P(k) of T.
 void enqueue_diff_forward(sf21_hii_ctx* c, const float* g, const float* emis, double zeta,
-                          int nbins, float* x) {
+                          int nbins, float* x, bool keep) {
   hipStream_t s = c->stream;
   enqueue_diff_fields(c, g, emis, zeta);
-  enqueue_fused(c, c->df_rec, c->df_ion, nullptr, 0.0, x, c->neutral_dev, -1, nullptr);
+  enqueue_fused(c, c->df_rec, c->df_ion, nullptr, 0.0, x, c->neutral_dev, -1, nullptr,
+                keep ? c->df_pairs : nullptr);
+  if (keep) launch(c, "fingerprint", [&] { launch_fingerprint(c->n3, g, emis, c->df_part, c->df_fp, s); });
   launch(c, "diff_t", [&] { launch_t_field(c->n3, x, c->df_delta, c->df_t, c->df_part, c->df_st, s); });
   enqueue_pk(c, c->df_t, c->df_half, nbins);
 }
 
 // Relaxed adjoint.  Expects df_gp (dL/dP_b) on the device.
+// kept: df_pairs hold this input's filtered pairs (kept by the forward);
+// sweep 1 reads them instead of recomputing them from the spectrum
 void enqueue_diff_backward(sf21_hii_ctx* c, const float* g, const float* emis, double zeta,
-                           double tau, int nbins, double gx, float* grad_g) {
+                           double tau, int nbins, double gx, float* grad_g, bool kept) {
   hipStream_t s = c->stream;
   const int n = c->n;
   const long n3 = c->n3;
@@ -528,22 +542,31 @@ void enqueue_diff_backward(sf21_hii_ctx* c, const float* g, const float* emis, d
   enqueue_diff_fields(c, g, emis, zeta);
   // forward spectrum Z of rec + i s ion, (p,j)-space
   launch(c, "balance", [&] { launch_balance(n3, c->df_rec, c->df_ion, nullptr, k, c->bal_part, c->bal, s); });
-  launch(c, "fwd_rows", [&] { launch_fwd_rows(n, c->df_rec, c->df_ion, nullptr, k, c->bal, c->spec, c->tw, s); });
-  launch(c, "cols_fwd_j", [&] { launch_cols(n, n, 0, -1, c->spec, c->spec, 1, 0, c->tw, s, 16, kColsStream); });
+  if (!kept) {
+    launch(c, "fwd_rows", [&] { launch_fwd_rows(n, c->df_rec, c->df_ion, nullptr, k, c->bal, c->spec, c->tw, s); });
+    launch(c, "cols_fwd_j", [&] { launch_cols(n, n, 0, -1, c->spec, c->spec, 1, 0, c->tw, s, 16, kColsStream); });
+  }
   // sweep 1: C = sum_R B_R FFT(phi_R)
   launch(c, "diff_fill", [&] { launch_fill_c(c->df_c, n3, s); });
   for (int rb = 0; rb < nr; rb += c->K) {
     const int kk = std::min(c->K, nr - rb);
     const int kp = (kk + 1) / 2;
-    launch(c, "filter_i", [&] {
-      launch_filter_i(n, c->spec, c->ubuf, kk, n3, c->wlut + (long)rb * c->mlut, c->mlut,
-                      c->d_kappa + rb, c->tw, s, c->tile_filter, c->filter_w, c->filter_pair);
-    });
-    launch(c, "cols_inv_j", [&] { launch_cols(n, n, 0, +1, c->ubuf, c->ubuf, kk, n3, c->tw, s, c->cols_tile, c->cols_wide | kColsStream); });
-    launch(c, "rows_phi", [&] {
-      launch_rows_phi(n, c->ubuf, kk, n3, c->bal, c->df_st, inv_tau, c->tw,
-                      c->df_pairs ? c->df_pairs + (long)rb * n3 : nullptr, n3, s);
-    });
+    if (kept) {
+      launch(c, "rows_phi", [&] {
+        launch_rows_phi(n, c->ubuf, kk, n3, c->bal, c->df_st, inv_tau, c->tw, c->df_pairs + (long)rb * n3,
+                        n3, s, true);
+      });
+    } else {
+      launch(c, "filter_i", [&] {
+        launch_filter_i(n, c->spec, c->ubuf, kk, n3, c->wlut + (long)rb * c->mlut, c->mlut,
+                        c->d_kappa + rb, c->tw, s, c->tile_filter, c->filter_w, c->filter_pair);
+      });
+      launch(c, "cols_inv_j", [&] { launch_cols(n, n, 0, +1, c->ubuf, c->ubuf, kk, n3, c->tw, s, c->cols_tile, c->cols_wide | kColsStream); });
+      launch(c, "rows_phi", [&] {
+        launch_rows_phi(n, c->ubuf, kk, n3, c->bal, c->df_st, inv_tau, c->tw,
+                        c->df_pairs ? c->df_pairs + (long)rb * n3 : nullptr, n3, s);
+      });
+    }
     launch(c, "cols_fwd_j", [&] { launch_cols(n, n, 0, -1, c->ubuf, c->ubuf, kp, n3, c->tw, s, c->cols_tile, c->cols_wide | kColsStream); });
     launch(c, "ball_acc_i", [&] {
       launch_ball_i(n, true, c->ubuf, n3, c->df_c, 0, kk, c->df_ball + (long)rb * h3, c->tw, s);
@@ -1590,10 +1613,11 @@ int sf21_fft_destroy(sf21_fft_plan* p) {
 
 // ---- differentiable mode ---------------------------------------------------
 int sf21_hii_diff_store_fields(sf21_hii_ctx* c, int mode) {
-  if (!c || mode < 0 || mode > 2) return SF21_ERR_ARG;
+  if (!c || mode < 0 || mode > 3) return SF21_ERR_ARG;
   if (hipSetDevice(c->cfg.device) != hipSuccess) return SF21_ERR_HIP;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   c->df_store = mode;
+  c->df_kept = false;
   if (mode == 0) {
     dev_free(c->df_pairs);
     return SF21_OK;
@@ -1602,18 +1626,33 @@ int sf21_hii_diff_store_fields(sf21_hii_ctx* c, int mode) {
   return dev_alloc(&c->df_pairs, (size_t)c->n3 * c->radii.size());
 }
 
+int sf21_hii_diff_last_kept(const sf21_hii_ctx* c) {
+  if (!c) return SF21_ERR_ARG;
+  return c->df_last_kept ? 1 : 0;
+}
+
 int sf21_hii_diff_forward_dev(sf21_hii_ctx* c, const float* g, const float* emis, double zeta,
                               int nbins, float* x_out, double* mean_x, double* pk) {
   int rc = diff_check(c, g, emis, nbins);
   if (rc) return rc;
   float* x = x_out ? x_out : c->df_x;
-  enqueue_diff_forward(c, g, emis, zeta, nbins, x);
+  const bool keep = c->df_store == 3 && c->df_pairs;
+  c->df_kept = false;
+  enqueue_diff_forward(c, g, emis, zeta, nbins, x, keep);
   if (hipMemcpyAsync(c->df_host, c->df_st, sizeof(double) * DS_COUNT, hipMemcpyDeviceToHost,
                      c->stream) != hipSuccess ||
       hipMemcpyAsync(c->df_host + DS_COUNT, c->pk_out, sizeof(double) * nbins,
                      hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+      (keep && hipMemcpyAsync(c->df_host + 200, c->df_fp, sizeof(double) * 2, hipMemcpyDeviceToHost,
+                              c->stream) != hipSuccess) ||
       hipStreamSynchronize(c->stream) != hipSuccess)
     return SF21_ERR_HIP;
+  if (keep) {
+    c->df_kept = true;
+    c->df_kept_fp[0] = c->df_host[200];
+    c->df_kept_fp[1] = c->df_host[201];
+    c->df_kept_zeta = zeta;
+  }
   if (mean_x) *mean_x = c->df_host[DS_SUM_X] / (double)c->n3;
   if (pk)
     for (int b = 0; b < nbins; ++b) pk[b] = c->df_host[DS_COUNT + b];
@@ -1632,7 +1671,24 @@ int sf21_hii_diff_backward_dev(sf21_hii_ctx* c, const float* g, const float* emi
   if (hipMemcpyAsync(c->df_gp, c->df_host, sizeof(double) * nbins, hipMemcpyHostToDevice,
                      c->stream) != hipSuccess)
     return SF21_ERR_HIP;
-  enqueue_diff_backward(c, g, emis, zeta, tau, nbins, dl_dx, grad_g);
+  // forward-kept fields: used when this call's (g, emis, zeta) fingerprint as
+  // the last forward's did; after a backward in mode 3 the pairs are this
+  // input's either way (sweep 1 rewrites them when it recomputes)
+  bool kept = false;
+  if (c->df_store == 3 && c->df_pairs) {
+    launch_fingerprint(c->n3, g, emis, c->df_part, c->df_fp + 2, c->stream);
+    if (hipMemcpyAsync(c->df_host + 202, c->df_fp + 2, sizeof(double) * 2, hipMemcpyDeviceToHost,
+                       c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+      return SF21_ERR_HIP;
+    kept = c->df_kept && zeta == c->df_kept_zeta && c->df_host[202] == c->df_kept_fp[0] &&
+           c->df_host[203] == c->df_kept_fp[1];
+    c->df_kept_fp[0] = c->df_host[202];
+    c->df_kept_fp[1] = c->df_host[203];
+    c->df_kept_zeta = zeta;
+  }
+  c->df_kept = false;  // set again below once the pairs are this input's
+  enqueue_diff_backward(c, g, emis, zeta, tau, nbins, dl_dx, grad_g, kept);
   // the staging buffer is reused for the results only after the stream has
   // consumed the upload above (same stream, ordered)
   if (hipMemcpyAsync(c->df_host + 64, c->df_st, sizeof(double) * DS_COUNT, hipMemcpyDeviceToHost,
@@ -1641,6 +1697,8 @@ int sf21_hii_diff_backward_dev(sf21_hii_ctx* c, const float* g, const float* emi
                      hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
       hipStreamSynchronize(c->stream) != hipSuccess)
     return SF21_ERR_HIP;
+  c->df_kept = c->df_store == 3 && c->df_pairs;
+  c->df_last_kept = kept;
   const double* hs = c->df_host + 64;
   if (grad_zeta) *grad_zeta = hs[DS_ZETA_BAR];
   if (relaxed_mean_x) *relaxed_mean_x = hs[DS_SUM_XT] / (double)c->n3;
