@@ -261,7 +261,13 @@ def bench_diff(torch, sf, n, side, radii, n_src, taus=(0.05, 0.02, 0.01), reps=3
             kf += tf
             kb[str(tau)] = round(tb * 1e3 / reps, 2)
         kf_ms = kf * 1e3 / (reps * len(taus))
+        ctx.reset_stats()
+        ctx.profile(True)
+        ctx.diff_backward(g, emis, zeta, taus[0], 1.0, cp, 20, grad_out=grad)
+        kst3 = ctx.kernel_stats()
+        ctx.profile(False)
         kept = {"forward_ms": round(kf_ms, 2), "backward_ms": kb, "backward_read_kept_fields": used,
+                "backward_kernels_ms": {k: [c, round(t, 3)] for k, (c, t) in kst3.items()},
                 "grad_over_forward": {k: round(v / kf_ms, 3) for k, v in kb.items()},
                 "forward_plus_backward_ms": {k: round(kf_ms + v, 2) for k, v in kb.items()},
                 "forward_plus_backward_ms_recompute": {k: round(fwd_ms + v, 2) for k, v in bwd.items()}}
