@@ -609,16 +609,27 @@ __global__ __launch_bounds__(256) void k_dt_rows_final(DtArgs a, const int* in,
     __syncthreads();
     int o[kQmax];
     if (valid) dt_line_wave<kQmax>(col, n, Q, a.h, lane, o);
-    if (valid && lane < n / Q)
-      for (int q = 0; q < Q; ++q) {
-        const long idx = row * n + Q * lane + q;
+    if (valid) {
+      // the lane's Q consecutive results go back into the wave's own column
+      // (the transform's reads of it are complete), then lane takes cells
+      // lane, lane + 64, ...: coalesced loads of the fields and stores of x
+      if (lane < n / Q)
+        for (int q = 0; q < Q; ++q) {
+          const int e = Q * lane + q;
+          col[e + (e >> 3)] = o[q] >= 0 ? 1 : 0;
+        }
+      lds_pin();
+      __builtin_amdgcn_wave_barrier();
+      for (int e = lane; e < n; e += 64) {
+        const long idx = row * n + e;
         float rec, ion, fres;
         cell_fields(k, pa, pb, pc, idx, rec, ion, fres);
-        float v = o[q] >= 0 ? 1.0f : quick_fill(rec, ion);
+        float v = col[e + (e >> 3)] ? 1.0f : quick_fill(rec, ion);
         v = fminf(v, 1.0f - fres);
         xhii[idx] = v;
         acc += 1.0 - (double)v;
       }
+    }
     lds_pin();
     __syncthreads();
   }
