@@ -759,6 +759,7 @@ void apply_choice(sf21_hii_ctx* c, const TuneChoice& t) {
   c->cols_wide = cols_variant(c->n, t.ct, t.cw);
   c->rows_wide = t.rw && wide_ok(c->n);
   c->filter_pair = t.fp && t.tile >= 8 && t.w;
+  c->df_kept = false;  // kept fields were computed with the previous choices
   drop_graphs(c);
 }
 
