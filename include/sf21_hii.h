@@ -50,6 +50,11 @@ extern "C" {
                                 two real inverse transforms per radius, ball FFT
                                 convolution (main loop) and stencil (small loop), the
                                 any-centre flag read back each radius */
+#define SF21_PATH_DEPOSIT 2  /* deposits only (sf21_sfrd_dev, sf21_nldensity_dev,
+                                sf21_hii_halo_emissivity_dev): any n >= 1 (n_halo a
+                                whole number of n-blocks as smooth_box takes it), no
+                                transforms and no bubble-finder buffers; radii may be
+                                NULL.  Every other entry point rejects such a context. */
 
 typedef struct sf21_hii_ctx sf21_hii_ctx;
 typedef struct sf21_params sf21_params;

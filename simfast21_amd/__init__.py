@@ -35,6 +35,7 @@ INPUT_DIRECT = 1
 INPUT_HALO = 2
 PATH_FUSED = 0
 PATH_PLAIN = 1
+PATH_DEPOSIT = 2  # deposits only: any n, no transforms (sfrd, nldensity, halo emissivity)
 
 FFT_R2C, FFT_C2R, FFT_C2C_FORWARD, FFT_C2C_BACKWARD = 0, 1, 2, 3
 

@@ -763,12 +763,48 @@ void apply_choice(sf21_hii_ctx* c, const TuneChoice& t) {
   drop_graphs(c);
 }
 
+// Deposit-only context: the stream and the parameters the deposit entry
+// points read; halo grid and sums are allocated on first use.
+int create_deposit(const sf21_hii_config* cfg, sf21_hii_ctx** out) {
+  if (cfg->n < 1 || cfg->n_halo < cfg->n || cfg->n_halo > 4096 || !(cfg->box_len > 0))
+    return SF21_ERR_ARG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= cfg->device || cfg->device < 0) {
+    (void)hipGetLastError();
+    return SF21_ERR_HIP;
+  }
+  if (hipSetDevice(cfg->device) != hipSuccess) return SF21_ERR_HIP;
+  auto* c = new sf21_hii_ctx();
+  c->cfg = *cfg;
+  c->cfg.radii = nullptr;
+  c->cfg.n_radii = 0;
+  c->n = cfg->n;
+  c->n3 = (long)c->n * c->n * c->n;
+  c->nh = (long)c->n * c->n * (c->n / 2 + 1);
+  c->L = cfg->box_len;
+  c->dx = c->L / c->n;
+  c->dk = 2 * kPi / c->L;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return SF21_ERR_HIP;
+  }
+  c->own_stream = true;
+  *out = c;
+  return SF21_OK;
+}
+
+// Entry points that need the transforms and the bubble finder's buffers.
+bool is_deposit(const sf21_hii_ctx* c) { return c && c->cfg.path == SF21_PATH_DEPOSIT; }
+
 }  // namespace
 
 extern "C" {
 
 int sf21_hii_create(const sf21_hii_config* cfg, sf21_hii_ctx** out) {
-  if (!cfg || !out || !cfg->radii || cfg->n_radii < 1 || !(cfg->box_len > 0)) return SF21_ERR_ARG;
+  if (!cfg || !out) return SF21_ERR_ARG;
+  *out = nullptr;
+  if (cfg->path == SF21_PATH_DEPOSIT) return create_deposit(cfg, out);
+  if (!cfg->radii || cfg->n_radii < 1 || !(cfg->box_len > 0)) return SF21_ERR_ARG;
   if (!size_supported(cfg->n, cfg->path)) return SF21_ERR_ARG;
   if (cfg->input_mode != SF21_INPUT_SFR && cfg->input_mode != SF21_INPUT_DIRECT &&
       cfg->input_mode != SF21_INPUT_HALO)
@@ -934,6 +970,7 @@ int sf21_hii_set_stream(sf21_hii_ctx* c, void* stream) {
 
 int sf21_hii_run_slice_dev(sf21_hii_ctx* c, const float* a, const float* b, const float* cc,
                            double z, float* xhii, double* neutral) {
+  if (is_deposit(c)) return SF21_ERR_ARG;
   if (!c || !a || !b || !xhii) return SF21_ERR_ARG;
   if (hipSetDevice(c->cfg.device) != hipSuccess) return SF21_ERR_HIP;
   (void)hipGetLastError();
@@ -955,6 +992,7 @@ int sf21_hii_run_slice_dev(sf21_hii_ctx* c, const float* a, const float* b, cons
 
 int sf21_hii_run_slice(sf21_hii_ctx* c, const float* a, const float* b, const float* cc, double z,
                        float* xhii, double* neutral) {
+  if (is_deposit(c)) return SF21_ERR_ARG;
   if (!c || !a || !b || !xhii) return SF21_ERR_ARG;
   if (hipSetDevice(c->cfg.device) != hipSuccess) return SF21_ERR_HIP;
   const size_t bytes = sizeof(float) * (size_t)c->n3;
@@ -978,6 +1016,7 @@ int sf21_hii_run_slice(sf21_hii_ctx* c, const float* a, const float* b, const fl
 int sf21_hii_run_batch_dev(sf21_hii_ctx* c, int ns, const float* const* a, const float* const* b,
                            const float* const* cc, const double* z, float* const* xhii,
                            double* neutral) {
+  if (is_deposit(c)) return SF21_ERR_ARG;
   if (!c || ns < 1 || !a || !b || !z || !xhii || ns > c->neutral_cap) return SF21_ERR_ARG;
   for (int s = 0; s < ns; ++s)
     if (!a[s] || !b[s] || !xhii[s]) return SF21_ERR_ARG;
@@ -1166,6 +1205,7 @@ int sf21_hii_run_multi_dev(int nbox, sf21_hii_ctx* const* ctx, const float* cons
 }
 
 int sf21_hii_weight_map(sf21_hii_ctx* c, int* host_out) {
+  if (is_deposit(c)) return SF21_ERR_ARG;
   if (!c || !host_out) return SF21_ERR_ARG;
   if (hipSetDevice(c->cfg.device) != hipSuccess) return SF21_ERR_HIP;
   if (hipStreamSynchronize(c->stream) != hipSuccess) return SF21_ERR_HIP;
@@ -1176,6 +1216,7 @@ int sf21_hii_weight_map(sf21_hii_ctx* c, int* host_out) {
 
 int sf21_hii_filtered_dev(sf21_hii_ctx* c, const float* a, const float* b, const float* cc,
                           double z, int radius_index, float* out) {
+  if (is_deposit(c)) return SF21_ERR_ARG;
   if (!c || !a || !b || !out || radius_index < 0 || radius_index >= (int)c->radii.size())
     return SF21_ERR_ARG;
   if (hipSetDevice(c->cfg.device) != hipSuccess) return SF21_ERR_HIP;
@@ -1188,6 +1229,7 @@ int sf21_hii_filtered_dev(sf21_hii_ctx* c, const float* a, const float* b, const
 
 int sf21_hii_autotune(sf21_hii_ctx* c, const float* a, const float* b, const float* cc, double z,
                       const char* cache_path, double* best_ms) {
+  if (is_deposit(c)) return SF21_ERR_ARG;
   if (!c || !a || !b || c->cfg.path != SF21_PATH_FUSED) return SF21_ERR_ARG;
   if (hipSetDevice(c->cfg.device) != hipSuccess) return SF21_ERR_HIP;
   (void)hipGetLastError();
@@ -1330,6 +1372,7 @@ int sf21_hii_reset_stats(sf21_hii_ctx* c) {
 // ---- binned power spectrum ------------------------------------------------
 int sf21_hii_power_spectrum_dev(sf21_hii_ctx* c, const float* field, int nbins, double* pk,
                                 double* kmean, double* counts) {
+  if (is_deposit(c)) return SF21_ERR_ARG;
   if (!c || !field || nbins < 1 || nbins > 64 || c->n < 64) return SF21_ERR_ARG;
   if (hipSetDevice(c->cfg.device) != hipSuccess) return SF21_ERR_HIP;
   (void)hipGetLastError();
@@ -1382,6 +1425,7 @@ int sf21_hii_halo_emissivity_dev(sf21_hii_ctx* c, const float* halos, long nhalo
 
 int sf21_hii_run_slice_halo(sf21_hii_ctx* c, const float* delta, const float* halos, long nhalos,
                             double z, float* xhii, double* neutral) {
+  if (is_deposit(c)) return SF21_ERR_ARG;
   if (!c || !delta || (!halos && nhalos > 0) || nhalos < 0 || !xhii) return SF21_ERR_ARG;
   if (c->cfg.input_mode != SF21_INPUT_HALO) return SF21_ERR_ARG;
   if (hipSetDevice(c->cfg.device) != hipSuccess) return SF21_ERR_HIP;
@@ -1412,6 +1456,7 @@ int sf21_hii_run_slice_halo(sf21_hii_ctx* c, const float* delta, const float* ha
 int sf21_t21_dev(sf21_hii_ctx* c, const sf21_params* p, double z, const float* delta,
                  const float* xhii, const float* xc, const float* xalpha, const float* tk,
                  float* out, double* mean_t21, double* mean_ts) {
+  if (is_deposit(c)) return SF21_ERR_ARG;
   if (!c || !p || !delta || !xhii || !out || c->n < 64) return SF21_ERR_ARG;
   const bool spin = xc || xalpha || tk;
   if (spin && !(xc && xalpha && tk)) return SF21_ERR_ARG;
@@ -1614,6 +1659,7 @@ int sf21_fft_destroy(sf21_fft_plan* p) {
 
 // ---- differentiable mode ---------------------------------------------------
 int sf21_hii_diff_store_fields(sf21_hii_ctx* c, int mode) {
+  if (is_deposit(c)) return SF21_ERR_ARG;
   if (!c || mode < 0 || mode > 3) return SF21_ERR_ARG;
   if (hipSetDevice(c->cfg.device) != hipSuccess) return SF21_ERR_HIP;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
@@ -1634,6 +1680,7 @@ int sf21_hii_diff_last_kept(const sf21_hii_ctx* c) {
 
 int sf21_hii_diff_forward_dev(sf21_hii_ctx* c, const float* g, const float* emis, double zeta,
                               int nbins, float* x_out, double* mean_x, double* pk) {
+  if (is_deposit(c)) return SF21_ERR_ARG;
   int rc = diff_check(c, g, emis, nbins);
   if (rc) return rc;
   float* x = x_out ? x_out : c->df_x;
@@ -1665,6 +1712,7 @@ int sf21_hii_diff_backward_dev(sf21_hii_ctx* c, const float* g, const float* emi
                                double tau, int nbins, double dl_dx, const double* dl_dpk,
                                float* grad_g, double* grad_zeta, double* relaxed_mean_x,
                                double* relaxed_pk) {
+  if (is_deposit(c)) return SF21_ERR_ARG;
   int rc = diff_check(c, g, emis, nbins);
   if (rc) return rc;
   if (!grad_g || !(tau > 0.0)) return SF21_ERR_ARG;

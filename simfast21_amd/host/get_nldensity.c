@@ -109,17 +109,14 @@ int main(int argc, char *argv[]) {
     exit(1);
   }
   {
+    /* deposits only: any N_smoothed, no transform buffers */
     sf21_hii_config cfg;
-    double r0 = p.dx;
     memset(&cfg, 0, sizeof(cfg));
     cfg.n = (int)p.n;
     cfg.box_len = p.box_len;
-    cfg.n_radii = 1;
-    cfg.radii = &r0;
     cfg.input_mode = SF21_INPUT_HALO;
     cfg.n_halo = (int)p.n_halo;
-    cfg.path = SF21_PATH_FUSED;
-    cfg.radius_batch = 1;
+    cfg.path = SF21_PATH_DEPOSIT;
     cfg.hubble = p.hubble;
     if ((rc = sf21_hii_create(&cfg, &ctx)) != SF21_OK) {
       printf("GPU setup failed: %s\n", sf21_strerror(rc));

@@ -67,8 +67,8 @@ def test_nldensity_vs_oracle(torch_cuda):
                float(np.sum(d0.astype(np.float64) * g_in + 1.0))) < 1e-4 * nh ** 3
 
 
-INI = """N_smoothed = 64
-N_halo = 128
+INI = """N_smoothed = {n}
+N_halo = {nh}
 sim_length = 100.0
 hubble = 0.7
 omega_matter = 0.3
@@ -82,13 +82,17 @@ Original_nldensity_box = T
 """
 
 
-def test_input_step_clis_vs_oracle(tmp_path, torch_cuda):
-    n, nh, lmpc = 64, 128, 100.0
+@pytest.mark.parametrize("n,nh", [(64, 128), (60, 120)])
+def test_input_step_clis_vs_oracle(tmp_path, torch_cuda, n, nh):
+    """The deposit CLIs on a base_dir; (60, 120) has the reference sample
+    ini's N_halo/N_smoothed = 1200/600 shape, sizes with no transform plan
+    (the deposits need none)."""
+    lmpc = 100.0
     L_h = lmpc * 0.7
     base = tmp_path / "run"
     for sub in ("Halos", "Velocity", "delta"):
         (base / sub).mkdir(parents=True)
-    (base / "simfast21.ini").write_text(INI)
+    (base / "simfast21.ini").write_text(INI.format(n=n, nh=nh))
     cats = {}
     for k, z in enumerate((10.0, 9.5)):
         cat = catalog(nh, 5000 + 1000 * k, 30 + k)
@@ -123,3 +127,36 @@ def test_input_step_clis_vs_oracle(tmp_path, torch_cuda):
         assert np.abs(got - rout).max() < 5e-5 and np.abs(goto - rbox).max() < 2e-4
     r2 = subprocess.run([str(bdir / "get_nldensity"), str(base)], capture_output=True, text=True)
     assert r2.stdout.count("already exists - skipping...") == 2
+
+
+@pytest.mark.parametrize("n,nh", [(60, 120), (75, 150)])
+def test_deposit_context_any_size(torch_cuda, n, nh):
+    """SF21_PATH_DEPOSIT: sizes without a transform plan, SFRD and the
+    non-linear density against the oracle; the finder's entry points refuse
+    such a context."""
+    torch = torch_cuda
+    L_h, z, h = 90.0, 8.5, 0.7
+    ctx = sf.HiiContext(n, L_h, [], sf.INPUT_HALO, sf.PATH_DEPOSIT, n_halo=nh)
+    cat = catalog(nh, 8000, 50 + n)
+    out, mean = ctx.sfrd(torch.from_numpy(cat).cuda(), z)
+    ref, rmean = orc.sfrd(cat, z, nh, n, L_h / nh)
+    err = np.abs(out.cpu().numpy() - ref) / ref.max()
+    assert err.max() < 2e-5 and abs(mean - rmean) <= 1e-5 * abs(rmean)
+    p = dict(hubble=h, omega_m=0.3, omega_b=0.045, omega_lambda=0.7, box_len=L_h, n=n, n_halo=nh)
+    d0 = smooth_field(nh, 60 + n, 2.0)
+    v = [smooth_field(nh, 61 + n + k, 6.0) for k in range(3)]
+    dev = lambda a: torch.from_numpy(a).cuda()
+    box = torch.empty(nh ** 3, dtype=torch.float32, device="cuda")
+    got, _ = ctx.nldensity(p, z, dev(v[0]), dev(v[1]), dev(v[2]), dev(d0), box_halo=box)
+    c = orc.cosmo(h, 0.3, 0.045, 0.7)
+    g_in = orc.growth(c, 300.0)
+    rbox, rout = orc.nldensity(nh, n, L_h, h, g_in, orc.growth(c, z) - g_in, v[0], v[1], v[2], d0)
+    print(f"\ndeposit context {n}/{nh}: sfrd {err.max():.2e}, nldensity "
+          f"{np.abs(box.cpu().numpy() - rbox).max():.2e} / {np.abs(got.cpu().numpy() - rout).max():.2e}",
+          flush=True)
+    assert np.abs(box.cpu().numpy() - rbox).max() < 2e-4
+    assert np.abs(got.cpu().numpy() - rout).max() < 5e-5
+    x = torch.zeros(n ** 3, dtype=torch.float32, device="cuda")
+    with pytest.raises(sf.Sf21Error):
+        ctx.run_slice(x, x)
+    ctx.close()
