@@ -186,14 +186,32 @@ __device__ __forceinline__ int fold_idx(int i, int n) { return i > (n >> 1) ? n 
 // ---------------------------------------------------------------------------
 // Relaxation helpers
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ float softplus_f(float v) {
-  return v > 20.0f ? v : (v < -30.0f ? expf(v) : log1pf(expf(v)));
-}
-__device__ __forceinline__ float sigmoid_f(float v) { return 1.0f / (1.0f + expf(-v)); }
-
+// phi(u) = softplus(u) - softplus(u - c) and phi'(u) = sigmoid(u) -
+// sigmoid(u - c), c = 20, each with one exponential t = e^u:
+//   phi  = log1p(t) - log1p(t e^-c), which is t (1 - e^-c) to FP32 precision
+//          below u = -17 (log1p(t) = t (1 - t/2 ...), t < 4.2e-8), and c
+//          above u = 2c (both softplus terms linear);
+//   phi' = t (1 - e^-c) / ((1 + t)(1 + t e^-c)) -- the difference of the two
+//          sigmoids without cancellation -- and e^(c-u) (1 - e^-c) above
+//          u = 2c, where t would overflow.
 constexpr float kPhiCap = 20.0f;
-__device__ __forceinline__ float phi_of(float u) { return softplus_f(u) - softplus_f(u - kPhiCap); }
-__device__ __forceinline__ float dphi_of(float u) { return sigmoid_f(u) - sigmoid_f(u - kPhiCap); }
+constexpr float kExpNegCap = 2.0611536224385583e-09f;  // e^-20
+// Contraction off: the same operations in every kernel that inlines them, so
+// the stored-field and recomputing backward passes agree bit for bit.
+__device__ __forceinline__ float phi_of(float u) {
+#pragma clang fp contract(off)
+  if (u > 2.0f * kPhiCap) return kPhiCap;
+  const float t = expf(u);
+  if (u < -17.0f) return t * (1.0f - kExpNegCap);
+  return log1pf(t) - log1pf(t * kExpNegCap);
+}
+__device__ __forceinline__ float dphi_of(float u) {
+#pragma clang fp contract(off)
+  if (u > 2.0f * kPhiCap) return expf(kPhiCap - u) * (1.0f - kExpNegCap);
+  const float t = expf(u);
+  const float den = (1.0f + t) * fmaf(t, kExpNegCap, 1.0f);
+  return t * (1.0f - kExpNegCap) / den;
+}
 
 // rho in unnormalised units: N rho = 1e-3 sum(rec)
 __device__ __forceinline__ float rho_u_of(const double* st) {
