@@ -120,11 +120,14 @@ def check_vs_oracle(torch, case, filter_w=0, dilate=orc.dilate_brute, label=""):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("n,zeta,filter_w", [(64, 0.3, 0), (64, 3.0, 0), (128, 1.0, 0),
-                                             (64, 3.0, 2), (128, 1.0, 2)])
+                                             (64, 3.0, 2), (128, 1.0, 2), (96, 1.0, 0),
+                                             (192, 1.0, 0)])
 def test_fused_vs_oracle(torch_cuda, n, zeta, filter_w):
     """filter_w 0: top-hat weights evaluated in FP32 in the kernel (the
-    default); 2: the FP64 table per integer |k|^2."""
+    default); 2: the FP64 table per integer |k|^2.  96 and 192 run the
+    radix-3 plans (12 values per thread) of the uneven-size batch."""
     check_vs_oracle(torch_cuda, direct_case(n, zeta), filter_w,
+                    dilate=orc.dilate_edt if n > 128 else orc.dilate_brute,
                     label=f"n={n} zeta={zeta} filter_w={filter_w}")
 
 
