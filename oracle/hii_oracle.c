@@ -314,6 +314,89 @@ void orc_dilate_weighted_edt(int n, const int *w, unsigned char *out) {
   free(dist);
 }
 
+/* One periodic line of the separable max-plus transform:
+ * g[x] = max over y with f[y] >= 0 of f[y] - dmin(x, y)^2, or -1 where that
+ * maximum is negative or no y is present.  Minimised form: d(x) = min over
+ * the images q = y - n, y, y + n of (x - q)^2 + h(q), h = -f, by the lower
+ * envelope of parabolas (Felzenszwalb and Huttenlocher, "Distance
+ * Transforms of Sampled Functions", 2012, Algorithm 1); for x in [0, n) the
+ * three images contain the min-image one. */
+static void maxplus_line(int n, const long *f, long *g, int *v, double *z) {
+  int k = -1;
+  for (int q = 0; q < 3 * n; q++) {
+    const long fq = f[q % n];
+    if (fq < 0) continue;
+    const double cq = (double)(q - n) * (double)(q - n) - (double)fq;
+    if (k < 0) {
+      k = 0;
+      v[0] = q;
+      z[0] = -HUGE_VAL;
+      z[1] = HUGE_VAL;
+      continue;
+    }
+    /* intersection with the envelope's last parabola, in line positions
+     * (image q sits at q - n); z[0] = -inf ends the loop at k = 0 */
+    double sx;
+    for (;;) {
+      const int r = v[k];
+      const double cr = (double)(r - n) * (double)(r - n) - (double)f[r % n];
+      sx = (cq - cr) / (2.0 * (double)(q - r));
+      if (sx <= z[k]) { k--; continue; }
+      break;
+    }
+    k++;
+    v[k] = q;
+    z[k] = sx;
+    z[k + 1] = HUGE_VAL;
+  }
+  if (k < 0) {
+    for (int x = 0; x < n; x++) g[x] = -1;
+    return;
+  }
+  int j = 0;
+  for (int x = 0; x < n; x++) {
+    while (z[j + 1] < (double)x) j++;
+    const long d = (long)(x + n - v[j]);
+    const long val = f[v[j] % n] - d * d;
+    g[x] = val >= 0 ? val : -1;
+  }
+}
+
+/* Same map as orc_dilate_weighted_brute: the separable max-plus transform
+ * g(x) = max_y (w(y) - |x-y|^2) one axis at a time (min-image squared
+ * distance is a sum of per-axis terms), out[x] = g(x) >= 0; a negative
+ * partial maximum cannot become non-negative in a later pass and is
+ * dropped.  O(N^3) for any number of distinct bounds. */
+void orc_dilate_weighted_maxplus(int n, const int *w, unsigned char *out) {
+  const long n3 = (long)n * n * n, nn = (long)n * n;
+  long *a = (long *)malloc(sizeof(long) * n3);
+  for (long i = 0; i < n3; i++) a[i] = w[i] >= 0 ? w[i] : -1;
+  for (int axis = 0; axis < 3; axis++) {
+    const long stride = axis == 0 ? 1 : (axis == 1 ? n : nn);
+#pragma omp parallel
+    {
+      long *f = (long *)malloc(sizeof(long) * n), *g = (long *)malloc(sizeof(long) * n);
+      int *v = (int *)malloc(sizeof(int) * 3 * n);
+      double *z = (double *)malloc(sizeof(double) * (3 * n + 1));
+#pragma omp for schedule(static)
+      for (long line = 0; line < nn; line++) {
+        /* line = (outer, inner) over the two other axes */
+        const long o = line / n, in = line % n;
+        long base;
+        if (axis == 0) base = o * nn + in * n;
+        else if (axis == 1) base = o * nn + in;
+        else base = o * n + in;
+        for (int e = 0; e < n; e++) f[e] = a[base + e * stride];
+        maxplus_line(n, f, g, v, z);
+        for (int e = 0; e < n; e++) a[base + e * stride] = g[e];
+      }
+      free(f); free(g); free(v); free(z);
+    }
+  }
+  for (long i = 0; i < n3; i++) out[i] = a[i] >= 0;
+  free(a);
+}
+
 /* ------------------------------------------------------------------ */
 /* Per-slice pipeline, once in the reference's typing (float fields,   */
 /* FP32 FFT, FP64 W and ratios) and once in FP64 throughout.           */

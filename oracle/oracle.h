@@ -90,6 +90,9 @@ void orc_dilate_weighted_brute(int n, const int *w, unsigned char *out);
 /* The same map by one squared Euclidean distance transform per distinct
  * bound (Felzenszwalb-Huttenlocher lower envelopes); for full-size boxes. */
 void orc_dilate_weighted_edt(int n, const int *w, unsigned char *out);
+/* The same map by the separable max-plus transform (per-line lower
+ * envelopes, three axes); O(N^3) whatever the number of distinct bounds. */
+void orc_dilate_weighted_maxplus(int n, const int *w, unsigned char *out);
 
 /* 20-bin spherically averaged power spectrum of a real box (FP64), linear
  * bins in |k| index space: see orc_power_spectrum in hii_oracle.c. */

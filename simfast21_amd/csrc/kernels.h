@@ -18,7 +18,38 @@ struct CellConsts {
 struct DtArgs {
   int n;  // box side
   int h;  // reach cap: min(floor(sqrt(max weight)), n/2)
+  int q;  // cells per lane in the line passes (n/q lanes, a multiple of 8, <= 64)
 };
+
+// Mixed-radix plan of a line length without a register plan (generic
+// sizes): Stockham stages of radix 4, 2, 3, 5, span ns[s] = product of the
+// earlier radices.
+struct GenPlan {
+  int L;
+  int nst;
+  int radix[16];
+  int ns[16];
+};
+
+// ---- generic sizes (lengths without a register plan) ----
+// Whole-line transforms through LDS, 8 lines per 256-thread block: axis 0
+// along p (contiguous rows), 1 along j, 2 along i (8 adjacent p columns).
+// mode 0: in place (src == dst allowed); 1: inverse-filter input -- each
+// element of src times W(|k| R) (kappa = dk R as hi + lo) before the
+// transform, result to dst; 2: centre test after the transform (Re = n^3
+// rec_R, Im = s n^3 ion_R, s = bal[0]): wmap = d2 where a centre, nothing
+// stored; 3: dst = (Re, Im / s) (the unscaled filtered pair).  LDS:
+// gen_pass_lds(L) bytes.
+bool gen_plan_make(int n, GenPlan* p);
+size_t gen_pass_lds(int L);
+// allows the transform blocks' LDS on the current device (false: refused)
+bool gen_prepare(int L);
+void launch_gen_pass(const GenPlan& plan, int axis, int sign, int mode, const float2* src,
+                     float2* dst, const float2* tw, float2 kappa, const float* bal, int* wmap,
+                     int d2, hipStream_t s);
+// spec = (rec, s ion) of the cell pre-pass, s = bal[0]
+void launch_gen_pack(long n3, const float* a, const float* b, const float* c, CellConsts k,
+                     const float* bal, float2* spec, hipStream_t s);
 
 // ---- fused exact path ----
 int balance_partials_count(long n3);
@@ -42,6 +73,8 @@ void launch_inv_rows_crit(int n, const float2* v, int nrad, long vstride, const 
                           const float2* tw, bool wide = false,
                           float2* keep = nullptr);
 void launch_fill_i32(int* p, int value, long count, hipStream_t s);
+// cells per lane of the dilation line passes for box side n (0: n not usable)
+int dt_q(int n);
 void launch_dt_rows(const DtArgs& a, const int* in, int* out, hipStream_t s);
 void launch_dt_cols(const DtArgs& a, const int* in, int* out, hipStream_t s);
 void launch_dt_final(const DtArgs& a, const int* in, const float* pa, const float* pb,

@@ -538,17 +538,21 @@ __device__ __forceinline__ void dt_line_wave(const int* col, int n, int Q, int h
 }
 
 constexpr int kDtCols = 16;  // columns per block in the strided passes
-constexpr int kQmax = 8;     // n <= 512
 
 __host__ __device__ constexpr int dt_col_floats(int n) { return n + n / 8 + 1; }
 
 // Strided pass: lines along i (axis 1: outer = j) or j (axis 0: outer = i)
 // for a tile of kDtCols adjacent p columns; in place through LDS.
+// QMAX >= a.q: the per-lane result array of the line pass.  The last tile
+// of a row may be partial (n a multiple of 8): its missing columns load as
+// empty (-1) lines and are not stored.
+template <int QMAX>
 __global__ __launch_bounds__(256) void k_dt_tile(DtArgs a, int axis, const int* in, int* out) {
   extern __shared__ int sm[];
-  const int n = a.n, Q = (n + 63) / 64, cs = dt_col_floats(n);
-  const int ntile = n / kDtCols;
+  const int n = a.n, Q = a.q, cs = dt_col_floats(n);
+  const int ntile = (n + kDtCols - 1) / kDtCols;
   const int outer = blockIdx.x / ntile, p0 = (blockIdx.x % ntile) * kDtCols;
+  const int ncol = n - p0 < kDtCols ? n - p0 : kDtCols;
   long base, stride;
   if (axis == 0) {
     base = (long)outer * n * n + p0;
@@ -559,15 +563,15 @@ __global__ __launch_bounds__(256) void k_dt_tile(DtArgs a, int axis, const int* 
   }
   for (int t = threadIdx.x; t < n * kDtCols; t += blockDim.x) {
     const int e = t / kDtCols, c = t % kDtCols;
-    sm[c * cs + e + (e >> 3)] = in[base + e * stride + c];
+    sm[c * cs + e + (e >> 3)] = c < ncol ? in[base + e * stride + c] : -1;
   }
   lds_pin();
   __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   for (int c = wave; c < kDtCols; c += blockDim.x >> 6) {
     int* col = sm + c * cs;
-    int o[kQmax];
-    dt_line_wave<kQmax>(col, n, Q, a.h, lane, o);
+    int o[QMAX];
+    dt_line_wave<QMAX>(col, n, Q, a.h, lane, o);
     if (lane < n / Q)
       for (int q = 0; q < Q; ++q) {
         const int e = Q * lane + q;
@@ -578,13 +582,14 @@ __global__ __launch_bounds__(256) void k_dt_tile(DtArgs a, int axis, const int* 
   __syncthreads();
   for (int t = threadIdx.x; t < n * kDtCols; t += blockDim.x) {
     const int e = t / kDtCols, c = t % kDtCols;
-    out[base + e * stride + c] = sm[c * cs + e + (e >> 3)];
+    if (c < ncol) out[base + e * stride + c] = sm[c * cs + e + (e >> 3)];
   }
 }
 
 // Pass along p, one row per wave, fused with the final per-cell value
 // (quick fill :277-288, union of balls, clamp :466-471) and FP64 partial
 // sums of 1 - xHII (:473-477).
+template <int QMAX>
 __global__ __launch_bounds__(256) void k_dt_rows_final(DtArgs a, const int* in,
                                                        const float* __restrict__ pa,
                                                        const float* __restrict__ pb,
@@ -592,7 +597,7 @@ __global__ __launch_bounds__(256) void k_dt_rows_final(DtArgs a, const int* in,
                                                        float* __restrict__ xhii,
                                                        double* __restrict__ partials) {
   extern __shared__ int sm[];
-  const int n = a.n, Q = (n + 63) / 64, cs = dt_col_floats(n);
+  const int n = a.n, Q = a.q, cs = dt_col_floats(n);
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
   int* col = sm + wave * cs;
   double acc = 0.0;
@@ -607,8 +612,8 @@ __global__ __launch_bounds__(256) void k_dt_rows_final(DtArgs a, const int* in,
     }
     lds_pin();
     __syncthreads();
-    int o[kQmax];
-    if (valid) dt_line_wave<kQmax>(col, n, Q, a.h, lane, o);
+    int o[QMAX];
+    if (valid) dt_line_wave<QMAX>(col, n, Q, a.h, lane, o);
     if (valid) {
       // the lane's Q consecutive results go back into the wave's own column
       // (the transform's reads of it are complete), then lane takes cells
@@ -656,6 +661,122 @@ __global__ __launch_bounds__(256) void k_sum_partials(const double* __restrict__
     __syncthreads();
   }
   if (threadIdx.x == 0) out[0] = red[0] * scale;
+}
+
+// ------------------------------------------------------------------------
+// Generic sizes (box sides with no register plan, e.g. 600 = 4*2*3*5*5):
+// whole-line mixed-radix Stockham transforms through LDS.  A block of 256
+// threads holds kGenLines lines of length L in two LDS buffers; each stage
+// reads its butterflies' inputs by butterfly index from one buffer and
+// writes the outputs to the other (radix R, span NS, for butterfly j < L/R:
+// k = j mod NS, a_r = x[j + r L/R] w^(k r), DFT_R, y[(j/NS) NS R + k + r NS]
+// = a_r -- the same self-sorting stages as the register transforms).
+// ------------------------------------------------------------------------
+constexpr int kGenLines = 8;
+
+template <int SIGN>
+__device__ __forceinline__ void gen_dft(float2 (&a)[5], int R) {
+  switch (R) {
+    case 2: Dft<2, SIGN>::run(a); break;
+    case 3: Dft<3, SIGN>::run(a); break;
+    case 4: Dft<4, SIGN>::run(a); break;
+    default: Dft<5, SIGN>::run(a); break;
+  }
+}
+
+// global index of element e of line l of block blk (axis 0: rows along p;
+// 1: columns along j at fixed i; 2: columns along i at fixed j)
+__device__ __forceinline__ long gen_index(int axis, int L, long blk, int l, int e) {
+  const long L2 = (long)L * L;
+  if (axis == 0) return (blk * kGenLines + l) * L + e;
+  const int ntile = L / kGenLines;
+  const long outer = blk / ntile;
+  const int p = (int)(blk % ntile) * kGenLines + l;
+  return axis == 1 ? outer * L2 + (long)e * L + p : (long)e * L2 + outer * L + p;
+}
+
+template <int SIGN, int MODE>
+__global__ __launch_bounds__(256) void k_gen_pass(const float2* src, float2* dst, GenPlan plan,
+                                                  int axis, const float2* __restrict__ tw,
+                                                  float2 kappa, const float* __restrict__ bal,
+                                                  int* __restrict__ wmap, int d2) {
+  extern __shared__ float2 gsm[];
+  const int L = plan.L;
+  const long blk = blockIdx.x;
+  float2* in = gsm;
+  float2* out = gsm + kGenLines * L;
+  // load (contiguous along p for axis 0, across the 8 columns otherwise)
+  for (int t = threadIdx.x; t < kGenLines * L; t += blockDim.x) {
+    const int l = axis == 0 ? t / L : t % kGenLines;
+    const int e = axis == 0 ? t % L : t / kGenLines;
+    const long idx = gen_index(axis, L, blk, l, e);
+    float2 v = src[idx];
+    if constexpr (MODE == 1) {
+      const long L2 = (long)L * L;
+      const int m = fold_sq((int)(idx / L2), L) + fold_sq((int)((idx / L) % L), L) +
+                    fold_sq((int)(idx % L), L);
+      const float w = w_tophat_k(kappa, m);
+      v = make_float2(v.x * w, v.y * w);
+    }
+    in[l * L + e] = v;
+  }
+  for (int st = 0; st < plan.nst; ++st) {
+    const int R = plan.radix[st], NS = plan.ns[st], Lr = L / R, tstep = L / (NS * R);
+    lds_pin();
+    __syncthreads();
+    for (int b = threadIdx.x; b < kGenLines * Lr; b += blockDim.x) {
+      const int l = b / Lr, j = b - l * Lr, k = j % NS;
+      const float2* x = in + l * L;
+      float2 a[5];
+#pragma unroll
+      for (int r = 0; r < 5; ++r) a[r] = r < R ? x[j + r * Lr] : make_float2(0.f, 0.f);
+      if (NS > 1) {
+#pragma unroll
+        for (int r = 1; r < 5; ++r)
+          if (r < R) {
+            float2 w = tw[k * r * tstep];
+            if (SIGN > 0) w.y = -w.y;
+            a[r] = cmul(a[r], w);
+          }
+      }
+      gen_dft<SIGN>(a, R);
+      float2* y = out + l * L + (j / NS) * NS * R + k;
+#pragma unroll
+      for (int r = 0; r < 5; ++r)
+        if (r < R) y[r * NS] = a[r];
+    }
+    float2* tmp = in;
+    in = out;
+    out = tmp;
+  }
+  lds_pin();
+  __syncthreads();
+  const float sc = MODE >= 2 ? bal[0] : 0.0f;
+  for (int t = threadIdx.x; t < kGenLines * L; t += blockDim.x) {
+    const int l = axis == 0 ? t / L : t % kGenLines;
+    const int e = axis == 0 ? t % L : t / kGenLines;
+    const long idx = gen_index(axis, L, blk, l, e);
+    const float2 v = in[l * L + e];
+    if constexpr (MODE == 2) {
+      if (is_centre(v.x * sc, v.y)) wmap[idx] = d2;
+    } else if constexpr (MODE == 3) {
+      dst[idx] = make_float2(v.x, v.y / sc);
+    } else {
+      dst[idx] = v;
+    }
+  }
+}
+
+__global__ void k_gen_pack(long n3, const float* __restrict__ a, const float* __restrict__ b,
+                           const float* __restrict__ c, CellConsts k,
+                           const float* __restrict__ bal, float2* __restrict__ spec) {
+  const float sc = bal[0];
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n3;
+       i += (long)gridDim.x * blockDim.x) {
+    float rec, ion, fr;
+    cell_fields(k, a, b, c, i, rec, ion, fr);
+    spec[i] = make_float2(rec, ion * sc);
+  }
 }
 
 // ------------------------------------------------------------------------
@@ -810,26 +931,102 @@ int dt_partials_count(int n) {
   return (int)(blocks < 2048 ? blocks : 2048);
 }
 
+int dt_q(int n) {
+  // cells per lane: n/q lanes, a multiple of 8 and at most 64 (0: none)
+  for (int q = 1; q <= 16; ++q)
+    if (n % q == 0 && n / q <= 64 && (n / q) % 8 == 0) return q;
+  return 0;
+}
+
 void launch_dt_rows(const DtArgs& a, const int* in, int* out, hipStream_t s) {
   // axis i: tiles (j, p-block)
   const size_t lds = sizeof(int) * kDtCols * dt_col_floats(a.n);
-  hipLaunchKernelGGL(k_dt_tile, dim3(a.n * (a.n / kDtCols)), dim3(256), lds, s, a, 1, in, out);
+  const dim3 g((unsigned)(a.n * ((a.n + kDtCols - 1) / kDtCols)));
+  if (a.q <= 8) hipLaunchKernelGGL(k_dt_tile<8>, g, dim3(256), lds, s, a, 1, in, out);
+  else hipLaunchKernelGGL(k_dt_tile<16>, g, dim3(256), lds, s, a, 1, in, out);
 }
 void launch_dt_cols(const DtArgs& a, const int* in, int* out, hipStream_t s) {
   // axis j: tiles (i, p-block)
   const size_t lds = sizeof(int) * kDtCols * dt_col_floats(a.n);
-  hipLaunchKernelGGL(k_dt_tile, dim3(a.n * (a.n / kDtCols)), dim3(256), lds, s, a, 0, in, out);
+  const dim3 g((unsigned)(a.n * ((a.n + kDtCols - 1) / kDtCols)));
+  if (a.q <= 8) hipLaunchKernelGGL(k_dt_tile<8>, g, dim3(256), lds, s, a, 0, in, out);
+  else hipLaunchKernelGGL(k_dt_tile<16>, g, dim3(256), lds, s, a, 0, in, out);
 }
 void launch_dt_final(const DtArgs& a, const int* in, const float* pa, const float* pb,
                      const float* pc, CellConsts k, float* xhii, double* partials,
                      int npartials, hipStream_t s) {
   const size_t lds = sizeof(int) * 4 * dt_col_floats(a.n);
-  hipLaunchKernelGGL(k_dt_rows_final, dim3(npartials), dim3(256), lds, s, a, in, pa, pb, pc, k,
-                     xhii, partials);
+  if (a.q <= 8)
+    hipLaunchKernelGGL(k_dt_rows_final<8>, dim3(npartials), dim3(256), lds, s, a, in, pa, pb, pc, k,
+                       xhii, partials);
+  else
+    hipLaunchKernelGGL(k_dt_rows_final<16>, dim3(npartials), dim3(256), lds, s, a, in, pa, pb, pc,
+                       k, xhii, partials);
 }
 void launch_sum_partials(const double* partials, int np, double scale, double* out,
                          hipStream_t s) {
   hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, partials, np, scale, out);
+}
+
+bool gen_plan_make(int n, GenPlan* p) {
+  if (n < 8 || n % kGenLines != 0) return false;
+  int m = n, ns = 1, st = 0;
+  while (m > 1) {
+    int r;
+    if (m % 4 == 0) r = 4;
+    else if (m % 2 == 0) r = 2;
+    else if (m % 3 == 0) r = 3;
+    else if (m % 5 == 0) r = 5;
+    else return false;
+    if (st >= 16) return false;
+    p->radix[st] = r;
+    p->ns[st] = ns;
+    ns *= r;
+    m /= r;
+    ++st;
+  }
+  p->L = n;
+  p->nst = st;
+  return true;
+}
+
+size_t gen_pass_lds(int L) { return sizeof(float2) * 2 * kGenLines * (size_t)L; }
+
+bool gen_prepare(int L) {
+  // LDS of the transform blocks above 64 KiB (L > 512) is requested per kernel
+  const size_t lds = gen_pass_lds(L);
+  if (lds <= 65536) return true;
+  const int v = (int)lds;
+  const hipFuncAttribute at = hipFuncAttributeMaxDynamicSharedMemorySize;
+  return hipFuncSetAttribute((const void*)k_gen_pass<-1, 0>, at, v) == hipSuccess &&
+         hipFuncSetAttribute((const void*)k_gen_pass<+1, 0>, at, v) == hipSuccess &&
+         hipFuncSetAttribute((const void*)k_gen_pass<+1, 1>, at, v) == hipSuccess &&
+         hipFuncSetAttribute((const void*)k_gen_pass<+1, 2>, at, v) == hipSuccess &&
+         hipFuncSetAttribute((const void*)k_gen_pass<+1, 3>, at, v) == hipSuccess;
+}
+
+void launch_gen_pass(const GenPlan& plan, int axis, int sign, int mode, const float2* src,
+                     float2* dst, const float2* tw, float2 kappa, const float* bal, int* wmap,
+                     int d2, hipStream_t s) {
+  const long L = plan.L;
+  const dim3 g((unsigned)(L * L / kGenLines)), b(256);
+  const size_t lds = gen_pass_lds(plan.L);
+#define GP(SG, MD)                                                                        \
+  hipLaunchKernelGGL((k_gen_pass<SG, MD>), g, b, lds, s, src, dst, plan, axis, tw, kappa, bal, \
+                     wmap, d2)
+  if (sign < 0) GP(-1, 0);
+  else if (mode == 1) GP(+1, 1);
+  else if (mode == 2) GP(+1, 2);
+  else if (mode == 3) GP(+1, 3);
+  else GP(+1, 0);
+#undef GP
+}
+
+void launch_gen_pack(long n3, const float* a, const float* b, const float* c, CellConsts k,
+                     const float* bal, float2* spec, hipStream_t s) {
+  long blocks = (n3 + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(k_gen_pack, dim3((unsigned)blocks), dim3(256), 0, s, n3, a, b, c, k, bal, spec);
 }
 
 }  // namespace sf21

@@ -44,12 +44,25 @@ double w_filter(double x) {
   return 3.0 * j;
 }
 
-bool size_supported(int n, int path) {
+// Box sides with register transform plans (32: complex transforms only).
+bool plan_size(int n) {
   switch (n) {
-    case 32: return path == SF21_PATH_FUSED;
     case 64: case 96: case 128: case 192: case 256: case 512: return true;
     default: return false;
   }
+}
+
+// Other sides of the fused finder: mixed-radix (2, 3, 5) transforms through
+// LDS, and dilation line passes whose lanes tile the side.
+bool generic_size(int n) {
+  GenPlan g;
+  return n >= 16 && n <= 1024 && gen_plan_make(n, &g) && dt_q(n) > 0;
+}
+
+bool size_supported(int n, int path) {
+  if (n == 32) return path == SF21_PATH_FUSED;
+  if (plan_size(n)) return true;
+  return path == SF21_PATH_FUSED && generic_size(n);
 }
 
 struct KStat {
@@ -87,6 +100,9 @@ struct sf21_hii_ctx {
   bool rows_wide = false;  // 16 values per thread in the centre-test pass
   int filter_w = 1;  // 1: FP32 on the fly, 0: FP64 table per |k|^2
   int filter_pair = 0;  // 1: the filter pass takes mirror pairs of i-lines
+  bool generic = false;  // fused finder on a side without a register plan
+  GenPlan gplan{};
+  std::vector<float2> kap;  // dk R per radius as a float pair (host copy)
   float2* d_kappa = nullptr;  // dk R per radius as a float pair (hi, lo)
   long n3 = 0, nh = 0;
   double L = 0, dx = 0, dk = 0;
@@ -265,11 +281,61 @@ int check_hip() {
   return e == hipSuccess ? SF21_OK : SF21_ERR_HIP;
 }
 
+// ---- fused exact path on a generic side: one radius per step ------------
+// Forward transforms of z = rec + i s ion along p, j, i; per radius in
+// ascending order, the filtered inverse along i, the inverse along j and the
+// inverse along p with the centre test; then the same dilation and final
+// pass as the register-plan sides.
+void enqueue_generic(sf21_hii_ctx* c, const float* a, const float* b, const float* cc, double z,
+                     float* xhii, double* neutral_slot, int only_radius, float2* dbg) {
+  const long n3 = c->n3;
+  const CellConsts k = cell_consts(c, z);
+  hipStream_t s = c->stream;
+  const GenPlan& g = c->gplan;
+  const float2 none = make_float2(0.f, 0.f);
+  launch(c, "balance", [&] { launch_balance(n3, a, b, cc, k, c->bal_part, c->bal, s); });
+  launch(c, "gen_pack", [&] { launch_gen_pack(n3, a, b, cc, k, c->bal, c->spec, s); });
+  for (int axis = 0; axis < 3; ++axis)
+    launch(c, "gen_fwd", [&] {
+      launch_gen_pass(g, axis, -1, 0, c->spec, c->spec, c->tw, none, nullptr, nullptr, 0, s);
+    });
+  launch(c, "fill_map", [&] { launch_fill_i32(c->wmap, -1, n3, s); });
+  const int r_lo = only_radius >= 0 ? only_radius : 0;
+  const int r_hi = only_radius >= 0 ? only_radius + 1 : (int)c->radii.size();
+  for (int r = r_hi - 1; r >= r_lo; --r) {
+    launch(c, "gen_filter_i", [&] {
+      launch_gen_pass(g, 2, +1, 1, c->spec, c->ubuf, c->tw, c->kap[r], nullptr, nullptr, 0, s);
+    });
+    launch(c, "gen_inv_j", [&] {
+      launch_gen_pass(g, 1, +1, 0, c->ubuf, c->ubuf, c->tw, none, nullptr, nullptr, 0, s);
+    });
+    launch(c, "gen_inv_p_crit", [&] {
+      if (dbg)
+        launch_gen_pass(g, 0, +1, 3, c->ubuf, dbg, c->tw, none, c->bal, nullptr, 0, s);
+      else
+        launch_gen_pass(g, 0, +1, 2, c->ubuf, nullptr, c->tw, none, c->bal, c->wmap, c->d2[r], s);
+    });
+  }
+  if (only_radius >= 0) return;
+  launch(c, "dt_rows", [&] { launch_dt_rows(c->dt, c->wmap, c->gbuf, s); });
+  launch(c, "dt_cols", [&] { launch_dt_cols(c->dt, c->gbuf, c->gbuf, s); });
+  launch(c, "dt_final", [&] {
+    launch_dt_final(c->dt, c->gbuf, a, b, cc, k, xhii, c->partials, c->npart, s);
+  });
+  launch(c, "sum", [&] {
+    launch_sum_partials(c->partials, c->npart, 1.0 / (double)n3, neutral_slot, s);
+  });
+}
+
 // ---- fused exact path: enqueue one slice -------------------------------
 // keep (nullable): the filtered pairs of every radius, radius r at keep + r n^3
 void enqueue_fused(sf21_hii_ctx* c, const float* a, const float* b, const float* cc, double z,
                    float* xhii, double* neutral_slot, int only_radius, float2* dbg,
                    float2* keep = nullptr) {
+  if (c->generic) {  // the differentiable mode (keep) refuses generic sides
+    enqueue_generic(c, a, b, cc, z, xhii, neutral_slot, only_radius, dbg);
+    return;
+  }
   const int n = c->n;
   const long n3 = c->n3;
   const CellConsts k = cell_consts(c, z);
@@ -795,6 +861,9 @@ int create_deposit(const sf21_hii_config* cfg, sf21_hii_ctx** out) {
 
 // Entry points that need the transforms and the bubble finder's buffers.
 bool is_deposit(const sf21_hii_ctx* c) { return c && c->cfg.path == SF21_PATH_DEPOSIT; }
+// Entry points built on the register transform plans (autotuner, power
+// spectrum, 21-cm step, differentiable mode).
+bool is_generic(const sf21_hii_ctx* c) { return c && c->generic; }
 
 }  // namespace
 
@@ -829,6 +898,12 @@ int sf21_hii_create(const sf21_hii_config* cfg, sf21_hii_ctx** out) {
   c->L = cfg->box_len;
   c->dx = c->L / c->n;
   c->dk = 2 * kPi / c->L;
+  c->generic = cfg->path == SF21_PATH_FUSED && !plan_size(c->n) && c->n != 32;
+  if (c->generic && (!gen_plan_make(c->n, &c->gplan) || !gen_prepare(c->n))) {
+    (void)hipGetLastError();
+    delete c;
+    return SF21_ERR_HIP;
+  }
   int rc = SF21_OK;
   long wmax = -1;
   for (double R : c->radii) {
@@ -844,8 +919,8 @@ int sf21_hii_create(const sf21_hii_config* cfg, sf21_hii_ctx** out) {
   c->K = cfg->radius_batch > 0 ? cfg->radius_batch : 16;
   bool tuned = false;
   TuneChoice tc;
-  if (cfg->path == SF21_PATH_FUSED && cfg->radius_batch <= 0 && cfg->filter_tile == 0 &&
-      cfg->filter_w == 0)
+  if (cfg->path == SF21_PATH_FUSED && !c->generic && cfg->radius_batch <= 0 &&
+      cfg->filter_tile == 0 && cfg->filter_w == 0)
     tuned = tune_lookup(tune_path(nullptr), tune_key(c), &tc);
   if (tuned) c->K = tc.K;
   if (cfg->filter_tile == 4 || cfg->filter_tile == 8 || cfg->filter_tile == 16)
@@ -862,6 +937,8 @@ int sf21_hii_create(const sf21_hii_config* cfg, sf21_hii_ctx** out) {
     c->filter_pair = tc.fp && tc.tile >= 8 && tc.w;
   }
   if (c->K > (int)c->radii.size()) c->K = (int)c->radii.size();
+  if (c->generic) c->K = 1;  // one radius per step, in-kernel FP32 weights
+  if (c->generic) c->filter_w = 1;
 
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     delete c;
@@ -888,6 +965,7 @@ int sf21_hii_create(const sf21_hii_config* cfg, sf21_hii_ctx** out) {
       const float hi = (float)k;
       kap[r] = make_float2(hi, (float)(k - (double)hi));
     }
+    c->kap = kap;
     if ((rc = dev_alloc(&c->d_kappa, kap.size()))) goto fail;
     if (hipMemcpy(c->d_kappa, kap.data(), sizeof(float2) * kap.size(), hipMemcpyHostToDevice) !=
         hipSuccess) {
@@ -916,6 +994,7 @@ int sf21_hii_create(const sf21_hii_config* cfg, sf21_hii_ctx** out) {
   {
     c->dt.n = c->n;
     c->dt.h = c->h;
+    c->dt.q = dt_q(c->n);
     c->npart = dt_partials_count(c->n);
     // the plain path uses the batch buffer only for the inspection hook
     if (cfg->path == SF21_PATH_PLAIN) c->K = 1;
@@ -1229,7 +1308,7 @@ int sf21_hii_filtered_dev(sf21_hii_ctx* c, const float* a, const float* b, const
 
 int sf21_hii_autotune(sf21_hii_ctx* c, const float* a, const float* b, const float* cc, double z,
                       const char* cache_path, double* best_ms) {
-  if (is_deposit(c)) return SF21_ERR_ARG;
+  if (is_deposit(c) || is_generic(c)) return SF21_ERR_ARG;
   if (!c || !a || !b || c->cfg.path != SF21_PATH_FUSED) return SF21_ERR_ARG;
   if (hipSetDevice(c->cfg.device) != hipSuccess) return SF21_ERR_HIP;
   (void)hipGetLastError();
@@ -1372,7 +1451,7 @@ int sf21_hii_reset_stats(sf21_hii_ctx* c) {
 // ---- binned power spectrum ------------------------------------------------
 int sf21_hii_power_spectrum_dev(sf21_hii_ctx* c, const float* field, int nbins, double* pk,
                                 double* kmean, double* counts) {
-  if (is_deposit(c)) return SF21_ERR_ARG;
+  if (is_deposit(c) || is_generic(c)) return SF21_ERR_ARG;
   if (!c || !field || nbins < 1 || nbins > 64 || c->n < 64) return SF21_ERR_ARG;
   if (hipSetDevice(c->cfg.device) != hipSuccess) return SF21_ERR_HIP;
   (void)hipGetLastError();
@@ -1456,7 +1535,7 @@ int sf21_hii_run_slice_halo(sf21_hii_ctx* c, const float* delta, const float* ha
 int sf21_t21_dev(sf21_hii_ctx* c, const sf21_params* p, double z, const float* delta,
                  const float* xhii, const float* xc, const float* xalpha, const float* tk,
                  float* out, double* mean_t21, double* mean_ts) {
-  if (is_deposit(c)) return SF21_ERR_ARG;
+  if (is_deposit(c) || is_generic(c)) return SF21_ERR_ARG;
   if (!c || !p || !delta || !xhii || !out || c->n < 64) return SF21_ERR_ARG;
   const bool spin = xc || xalpha || tk;
   if (spin && !(xc && xalpha && tk)) return SF21_ERR_ARG;
@@ -1593,7 +1672,7 @@ extern "C" {
 int sf21_fft_plan_3d(int n, int kind, sf21_fft_plan** out) {
   if (!out || kind < SF21_FFT_R2C || kind > SF21_FFT_C2C_BACKWARD) return SF21_ERR_ARG;
   const bool real = kind == SF21_FFT_R2C || kind == SF21_FFT_C2R;
-  if (!size_supported(n, real ? SF21_PATH_PLAIN : SF21_PATH_FUSED)) return SF21_ERR_ARG;
+  if (!plan_size(n) && !(n == 32 && !real)) return SF21_ERR_ARG;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
     (void)hipGetLastError();
@@ -1659,7 +1738,7 @@ int sf21_fft_destroy(sf21_fft_plan* p) {
 
 // ---- differentiable mode ---------------------------------------------------
 int sf21_hii_diff_store_fields(sf21_hii_ctx* c, int mode) {
-  if (is_deposit(c)) return SF21_ERR_ARG;
+  if (is_deposit(c) || is_generic(c)) return SF21_ERR_ARG;
   if (!c || mode < 0 || mode > 3) return SF21_ERR_ARG;
   if (hipSetDevice(c->cfg.device) != hipSuccess) return SF21_ERR_HIP;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
@@ -1680,7 +1759,7 @@ int sf21_hii_diff_last_kept(const sf21_hii_ctx* c) {
 
 int sf21_hii_diff_forward_dev(sf21_hii_ctx* c, const float* g, const float* emis, double zeta,
                               int nbins, float* x_out, double* mean_x, double* pk) {
-  if (is_deposit(c)) return SF21_ERR_ARG;
+  if (is_deposit(c) || is_generic(c)) return SF21_ERR_ARG;
   int rc = diff_check(c, g, emis, nbins);
   if (rc) return rc;
   float* x = x_out ? x_out : c->df_x;
@@ -1712,7 +1791,7 @@ int sf21_hii_diff_backward_dev(sf21_hii_ctx* c, const float* g, const float* emi
                                double tau, int nbins, double dl_dx, const double* dl_dpk,
                                float* grad_g, double* grad_zeta, double* relaxed_mean_x,
                                double* relaxed_pk) {
-  if (is_deposit(c)) return SF21_ERR_ARG;
+  if (is_deposit(c) || is_generic(c)) return SF21_ERR_ARG;
   int rc = diff_check(c, g, emis, nbins);
   if (rc) return rc;
   if (!grad_g || !(tau > 0.0)) return SF21_ERR_ARG;

@@ -54,6 +54,7 @@ def lib():
         L.orc_filtered_f.argtypes = L.orc_filtered_d.argtypes
         L.orc_dilate_weighted_brute.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
         L.orc_dilate_weighted_edt.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_dilate_weighted_maxplus.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
         L.orc_power_spectrum.argtypes = [ctypes.c_int, d, ctypes.c_void_p, ctypes.c_int,
                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         L.orc_halo_emissivity.argtypes = [ctypes.c_long, ctypes.c_void_p, d, ctypes.c_long, ctypes.c_long,
@@ -111,6 +112,13 @@ def dilate_brute(n, w):
     w = np.ascontiguousarray(w, np.int32).ravel()
     out = np.zeros(n ** 3, np.uint8)
     lib().orc_dilate_weighted_brute(n, _p(w), _p(out))
+    return out
+
+
+def dilate_maxplus(n, w):
+    w = np.ascontiguousarray(w, np.int32).ravel()
+    out = np.zeros(n ** 3, np.uint8)
+    lib().orc_dilate_weighted_maxplus(n, _p(w), _p(out))
     return out
 
 

@@ -1,0 +1,123 @@
+"""Fused finder on box sides without a register transform plan (multiples
+of 8 with prime factors 2, 3 and 5): mixed-radix transforms through LDS,
+one radius per step, then the same dilation and final pass as the plan
+sides.
+
+  * 80 (4*4*5), 120 (4*2*3*5), 200 (4*2*5*5): the fused parity bar against
+    the FP64 oracle (the xHII box equals the exact dilation of the GPU's own
+    centre map merged with the quick fill, bit for bit; centre-map
+    differences only at near-threshold cells; mean x_HII);
+  * 600 (4*2*3*5*5, the N_smoothed of the reference's sample
+    simfast21.ini; dilation lanes of 15 cells): a plane wave through the
+    filtered-field hook equals W(|k| R) times the wave (closed form), and
+    the xHII box of a synthetic slice equals the exact dilation of the
+    GPU's own centre map merged with the quick fill, bit for bit, with the
+    neutral fraction its FP64 mean."""
+import numpy as np
+import pytest
+
+import simfast21_amd as sf
+from helpers import direct_case
+import oracle_bind as orc
+from test_gpu_fused import check_vs_oracle, quick_fill_np
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,zeta", [(80, 1.0), (120, 1.0), (120, 3.0), (200, 1.0)])
+def test_generic_vs_oracle(torch_cuda, n, zeta):
+    case = direct_case(n, zeta)
+    ctx = sf.HiiContext(n, case["side"], case["radii"], sf.INPUT_DIRECT, sf.PATH_FUSED)
+    assert ctx.radius_batch == 1
+    ctx.close()
+    check_vs_oracle(torch_cuda, case, dilate=orc.dilate_maxplus, label=f"generic n={n} zeta={zeta}")
+
+
+def w_tophat64(x):
+    x = np.asarray(x, np.float64)
+    return np.where(x < 1e-3, 1.0 - x * x / 10.0, 3.0 * (np.sin(x) - x * np.cos(x)) / x ** 3)
+
+
+@pytest.mark.gpu
+def test_generic_600_plane_wave(torch_cuda):
+    """rec = 1 + 0.5 cos(2 pi k1.x / n) + 0.3 cos(2 pi k3.x / n), ion = 0.25
+    + 0.4 cos(2 pi k2.x / n): the filtered pair at radius R is n^3 (1 + 0.5
+    W(|k1| dk R) cos1 + 0.3 W(|k3| dk R) cos3, 0.25 + 0.4 W(|k2| dk R) cos2),
+    W(0) = 1.  k2 = (597, 4, 590) folds to (-3, 4, -10); k3 = (250, 300, 7)
+    holds the Nyquist index 300.  Within 1e-5 of the field maximum."""
+    torch = torch_cuda
+    n, side = 600, 600.0
+    radii = [40.0, 9.0, 2.5]
+    i = np.arange(n, dtype=np.float64)
+    k1, k2, k3 = (3, 7, 2), (597, 4, 590), (250, 300, 7)
+
+    def wave(k):
+        ph = (2 * np.pi / n) * ((k[0] * i)[:, None, None] + (k[1] * i)[None, :, None]
+                                + (k[2] * i)[None, None, :])
+        return np.cos(ph)
+
+    c1, c2, c3 = wave(k1), wave(k2), wave(k3)
+    rec = (1.0 + 0.5 * c1 + 0.3 * c3).astype(np.float32)
+    ion = (0.25 + 0.4 * c2).astype(np.float32)
+    ctx = sf.HiiContext(n, side, radii, sf.INPUT_DIRECT, sf.PATH_FUSED)
+    a = torch.from_numpy(rec.ravel()).cuda()
+    b = torch.from_numpy(ion.ravel()).cuda()
+    dk = 2 * np.pi / side
+    kk1 = np.sqrt(sum(min(q, n - q) ** 2 for q in k1)) * dk
+    kk2 = np.sqrt(sum(min(q, n - q) ** 2 for q in k2)) * dk
+    kk3 = np.sqrt(sum(min(q, n - q) ** 2 for q in k3)) * dk
+    for r, R in enumerate(radii):
+        g = ctx.filtered(a, b, radius_index=r).cpu().numpy().astype(np.float64) / n ** 3
+        er = 1.0 + 0.5 * w_tophat64(kk1 * R) * c1.ravel() + 0.3 * w_tophat64(kk3 * R) * c3.ravel()
+        ei = 0.25 + 0.4 * w_tophat64(kk2 * R) * c2.ravel()
+        err_r = np.abs(g[:, 0] - er).max() / np.abs(er).max()
+        err_i = np.abs(g[:, 1] - ei).max() / np.abs(ei).max()
+        print(f"R={R}: W1 {w_tophat64(kk1 * R):.6f} W2 {w_tophat64(kk2 * R):.6f} "
+              f"W3 {w_tophat64(kk3 * R):.2e} "
+              f"rel err rec {err_r:.2e} ion {err_i:.2e}", flush=True)
+        assert err_r < 1e-5 and err_i < 1e-5
+    ctx.close()
+
+
+@pytest.mark.gpu
+def test_generic_600_dilation_and_mean(torch_cuda):
+    torch = torch_cuda
+    n, side = 600, 937.5
+    delta, emis, rec = sf.synth.direct_boxes(n, side, 20000, 7)
+    ion = emis.astype(np.float32)
+    radii = sf.synth.geometric_ladder(281.25, 0.9, side / n)
+    ctx = sf.HiiContext(n, side, radii, sf.INPUT_DIRECT, sf.PATH_FUSED)
+    a = torch.from_numpy(rec.ravel()).cuda()
+    b = torch.from_numpy(ion.ravel()).cuda()
+    x, neu = ctx.run_slice(a, b)
+    torch.cuda.synchronize()
+    x = x.cpu().numpy()
+    w = ctx.weight_map()
+    ctx.close()
+    ncentre = int(np.count_nonzero(w >= 0))
+    print(f"n=600: {len(radii)} radii, centres {ncentre}, mean x_HII {1 - neu:.6f}", flush=True)
+    assert 0 < ncentre < n ** 3
+    dil = orc.dilate_maxplus(n, w).astype(bool)
+    expect = np.where(dil, np.float32(1.0), quick_fill_np(rec.ravel(), ion.ravel()))
+    assert np.array_equal(x, expect)
+    assert abs((1.0 - x.astype(np.float64).mean()) - neu) < 1e-9
+
+
+@pytest.mark.gpu
+def test_generic_refusals(torch_cuda):
+    """Entry points built on the register plans refuse generic sides with
+    an argument error instead of running."""
+    torch = torch_cuda
+    n = 80
+    case = direct_case(n, 1.0)
+    ctx = sf.HiiContext(n, case["side"], case["radii"], sf.INPUT_DIRECT, sf.PATH_FUSED)
+    a = torch.from_numpy(case["rec"].ravel()).cuda()
+    b = torch.from_numpy(case["ion"].ravel()).cuda()
+    with pytest.raises(sf.Sf21Error):
+        ctx.autotune(a, b)
+    with pytest.raises(sf.Sf21Error):
+        ctx.power_spectrum(a)
+    with pytest.raises(sf.Sf21Error):
+        ctx.diff_forward(a, b, 1.0)
+    x, neu = ctx.run_slice(a, b)  # the context still runs
+    assert 0.0 <= neu <= 1.0
+    ctx.close()

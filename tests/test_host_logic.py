@@ -115,6 +115,26 @@ def test_create_without_gpu_fails_cleanly():
     assert sf._lib().sf21_hii_create(ctypes.byref(cfg), ctypes.byref(h)) == sf.SF21_ERR_ARG
 
 
+@pytest.mark.parametrize("n,path,ok", [
+    (600, "FUSED", True), (80, "FUSED", True), (120, "FUSED", True), (200, "FUSED", True),
+    (360, "FUSED", True), (1024, "FUSED", True), (600, "PLAIN", False), (56, "FUSED", False),
+    (100, "FUSED", False), (1040, "FUSED", False), (1000, "FUSED", False), (8, "FUSED", False)])
+def test_size_acceptance(n, path, ok):
+    """Sides the fused finder takes: the register-plan sides and any other
+    multiple of 8 from 16 to 1024 with prime factors 2, 3 and 5 whose
+    dilation lanes tile it (1000 = 8 * 125 does not: no q <= 16 with n/q a
+    multiple of 8 and at most 64).  The plain path keeps the plan sides.
+    Without a GPU an accepted size fails later, at device selection."""
+    if _have_gpu():
+        pytest.skip("a HIP device is present")
+    radii = (ctypes.c_double * 2)(4.0, 2.0)
+    cfg = sf.HiiConfig(n, 100.0, 2, radii, sf.INPUT_DIRECT, getattr(sf, "PATH_" + path), 0.7, 1.0,
+                       0, 0, 0, 0, 0)
+    h = ctypes.c_void_p()
+    rc = sf._lib().sf21_hii_create(ctypes.byref(cfg), ctypes.byref(h))
+    assert rc == (sf.SF21_ERR_HIP if ok else sf.SF21_ERR_ARG)
+
+
 @pytest.mark.parametrize("cli", ["product", "oracle"])
 def test_cli_usage_and_errors(tmp_path, cli):
     exe = str(sf.CLI_PATH) if cli == "product" else orc.CLI

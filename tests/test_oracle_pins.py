@@ -125,8 +125,25 @@ def test_edt_dilation_equals_brute_force(n, seed):
         w[(face * n + n // 3) * n + 2] = 4
         w[(3 * n + face) * n + n // 2] = 6
         w[(n // 2 * n + 1) * n + face] = 2
-    assert np.array_equal(orc.dilate_edt(n, w), orc.dilate_brute(n, w))
-    assert orc.dilate_edt(n, np.full(n ** 3, -1, np.int32)).sum() == 0
+    brute = orc.dilate_brute(n, w)
+    assert np.array_equal(orc.dilate_edt(n, w), brute)
+    assert np.array_equal(orc.dilate_maxplus(n, w), brute)
+    empty = np.full(n ** 3, -1, np.int32)
+    assert orc.dilate_edt(n, empty).sum() == 0 and orc.dilate_maxplus(n, empty).sum() == 0
+
+
+@pytest.mark.parametrize("n,seed", [(18, 5), (30, 9)])
+def test_maxplus_dilation_equals_brute_force(n, seed):
+    """The max-plus dilation (one pass per axis, any number of distinct
+    bounds) against brute force on 20 random maps each, odd half sides and
+    bounds past the half box included."""
+    rng = np.random.default_rng(seed)
+    for _ in range(20):
+        w = np.full(n ** 3, -1, np.int32)
+        k = int(rng.integers(1, 40))
+        cells = rng.choice(n ** 3, k, replace=False)
+        w[cells] = rng.choice([0, 1, 2, 3, 5, 7, 9, 14, 30, 50, (n // 2) ** 2, 3 * (n // 2) ** 2], k)
+        assert np.array_equal(orc.dilate_maxplus(n, w), orc.dilate_brute(n, w))
 
 
 def test_oracle_uniform_known_answers():
