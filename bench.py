@@ -302,6 +302,45 @@ def bench_diff(torch, sf, n, side, radii, n_src, taus=(0.05, 0.02, 0.01), reps=3
     }
 
 
+def bench_generic(torch, sf, synth, n=600, steps=2, seed=6006):
+    """One slice at a box side without a register plan (600 = 4*2*3*5*5, the
+    N_smoothed of SimFast21's sample ini): the same recipe scaled to n, one
+    radius per step through LDS line transforms."""
+    side = 400.0 * n / 512
+    radii = synth.geometric_ladder(100.0 * n / 512, 0.9, side / n)
+    n_src = int(round(2.5e5 * (n / 512) ** 3))
+    recs, ions = make_inputs(torch, n, side, 3, n_src, seed=seed)
+    recs, ions = recs[1:2], ions[1:2]  # the middle growth factor of the batch recipe
+    torch.cuda.empty_cache()
+    ctx = sf.HiiContext(n, side, radii, sf.INPUT_DIRECT, sf.PATH_FUSED)
+    out = torch.empty_like(recs[0])
+    ctx.run_slice(recs[0], ions[0], xhii=out)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        _, neu = ctx.run_slice(recs[0], ions[0], xhii=out)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    ctx.reset_stats()
+    ctx.profile(True)
+    ctx.run_slice(recs[0], ions[0], xhii=out)
+    torch.cuda.synchronize()
+    stats = ctx.kernel_stats()
+    ctx.profile(False)
+    ctx.close()
+    v = n ** 3
+    algo = {"gen_filter_i": 16 * v, "gen_inv_j": 16 * v, "gen_inv_p_crit": 8 * v, "gen_fwd": 16 * v}
+    kern = {}
+    for k, (c, t) in stats.items():
+        e = {"launches": c, "ms": round(t, 3), "avg_launch_ms": round(t / c, 4)}
+        if k in algo:
+            e["hbm_frac"] = round(algo[k] / (t / c * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+        kern[k] = e
+    return {"n": n, "side": side, "radii": len(radii), "ms_per_slice": round(ms, 2),
+            "voxel_radius_per_s": v * len(radii) / (ms * 1e-3), "mean_xhii": round(1 - neu, 6),
+            "algorithmic_bytes_per_voxel_radius": 40, "kernels": kern}
+
+
 def cpu_baseline(n, side, radii_full, seconds_hint=20.0):
     """Oracle (CPU restatement, reference dataflow, FP32 typing) on a bounded
     sample of the workload: one 512^3 slice with the first radii of the ladder."""
@@ -345,6 +384,7 @@ def main():
     ap.add_argument("--no-diff", action="store_true")
     ap.add_argument("--no-uneven", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--no-generic", action="store_true")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one 1 GiB device copy (torch copy_) as the HBM-counter reference")
     args = ap.parse_args()
@@ -477,6 +517,15 @@ def main():
         print(f"[bench] diff forward {diff['forward_ms']:.1f} ms, grad/forward "
               f"{diff['grad_over_forward']}", file=sys.stderr, flush=True)
 
+    generic = None
+    if not args.no_generic:
+        try:
+            generic = bench_generic(torch, sf, synth)
+        finally:
+            torch.cuda.empty_cache()
+        print(f"[bench] generic side {generic['n']}: {generic['ms_per_slice']:.1f} ms/slice, "
+              f"{generic['voxel_radius_per_s']:.3g} voxel-radius/s", file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu:
         try:
@@ -510,6 +559,7 @@ def main():
         "plain": plain,
         "uneven_batch": uneven,
         "diff": diff,
+        "generic_side": generic,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -61,7 +61,7 @@ typedef struct sf21_params sf21_params;
 
 typedef struct sf21_hii_config {
   int n;                 /* N_smoothed: 64, 96, 128, 192, 256, 512 (32 fused only); fused
-                            path also any other multiple of 8 in 16..1024 with factors
+                            path also any other multiple of 8 in 16..768 with factors
                             2, 3, 5 and n/q a multiple of 8, at most 64, for some q <= 16
                             (e.g. 80, 120, 200, 360, 600); those sides refuse the
                             autotuner, power spectrum, t21 and differentiable mode */

@@ -22,28 +22,31 @@ struct DtArgs {
 };
 
 // Mixed-radix plan of a line length without a register plan (generic
-// sizes): Stockham stages of radix 4, 2, 3, 5, span ns[s] = product of the
-// earlier radices.
+// sizes): Stockham stages of radix 8, 4, 2, 3, 5, span ns[s] = product of
+// the earlier radices.
 struct GenPlan {
   int L;
   int nst;
+  int lp;          // LDS line stride (float2)
   int radix[16];
   int ns[16];
+  float rns[16];   // 1 / ns in FP32 (exact quotients of the lane indices)
 };
 
 // ---- generic sizes (lengths without a register plan) ----
-// Whole-line transforms through LDS, 8 lines per 256-thread block: axis 0
+// Whole-line transforms (first and last stage in HBM, the others through
+// LDS), 8 lines per 256-thread block: axis 0
 // along p (contiguous rows), 1 along j, 2 along i (8 adjacent p columns).
 // mode 0: in place (src == dst allowed); 1: inverse-filter input -- each
 // element of src times W(|k| R) (kappa = dk R as hi + lo) before the
 // transform, result to dst; 2: centre test after the transform (Re = n^3
 // rec_R, Im = s n^3 ion_R, s = bal[0]): wmap = d2 where a centre, nothing
 // stored; 3: dst = (Re, Im / s) (the unscaled filtered pair).  LDS:
-// gen_pass_lds(L) bytes.
+// gen_pass_lds(plan) bytes.
 bool gen_plan_make(int n, GenPlan* p);
-size_t gen_pass_lds(int L);
+size_t gen_pass_lds(const GenPlan& p);
 // allows the transform blocks' LDS on the current device (false: refused)
-bool gen_prepare(int L);
+bool gen_prepare(const GenPlan& p);
 void launch_gen_pass(const GenPlan& plan, int axis, int sign, int mode, const float2* src,
                      float2* dst, const float2* tw, float2 kappa, const float* bal, int* wmap,
                      int d2, hipStream_t s);

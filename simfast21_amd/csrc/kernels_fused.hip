@@ -664,107 +664,187 @@ __global__ __launch_bounds__(256) void k_sum_partials(const double* __restrict__
 }
 
 // ------------------------------------------------------------------------
-// Generic sizes (box sides with no register plan, e.g. 600 = 4*2*3*5*5):
-// whole-line mixed-radix Stockham transforms through LDS.  A block of 256
-// threads holds kGenLines lines of length L in two LDS buffers; each stage
-// reads its butterflies' inputs by butterfly index from one buffer and
-// writes the outputs to the other (radix R, span NS, for butterfly j < L/R:
-// k = j mod NS, a_r = x[j + r L/R] w^(k r), DFT_R, y[(j/NS) NS R + k + r NS]
-// = a_r -- the same self-sorting stages as the register transforms).
+// Generic sizes (box sides with no register plan, e.g. 600 = 8*3*5*5):
+// whole-line mixed-radix Stockham transforms, 8 lines per 256-thread block.
+// Stage of radix R and span NS (the product of the earlier radices),
+// butterfly j < L/R with k = j mod NS: a_r = x[j + r L/R] w^(k r), w =
+// exp(-+2 pi i / (NS R)); DFT_R; y[(j/NS) NS R + k + r NS] = a_r -- the
+// self-sorting stages of the register transforms.  The first stage reads its
+// inputs from HBM and the last writes its outputs there (positions j + r L/R
+// in both, so runs of consecutive j stay coalesced); the stages in between
+// exchange through one padded LDS line per line (a thread's butterflies are
+// read into registers, barrier, written, barrier).  Lanes: rows (axis 0)
+// take 32 consecutive butterflies of one line per half wave; the column
+// passes (axes 1, 2) take the 8 lines -- adjacent p columns, 64 B runs --
+// fastest.  32-bit indices (L^3 <= 2^30).
 // ------------------------------------------------------------------------
 constexpr int kGenLines = 8;
+constexpr int kGenMaxL = 768;
 
-template <int SIGN>
-__device__ __forceinline__ void gen_dft(float2 (&a)[5], int R) {
-  switch (R) {
-    case 2: Dft<2, SIGN>::run(a); break;
-    case 3: Dft<3, SIGN>::run(a); break;
-    case 4: Dft<4, SIGN>::run(a); break;
-    default: Dft<5, SIGN>::run(a); break;
+// floor(a / d) for 0 <= a, d <= 2^11 from rd = 1/d rounded to float:
+// (a + 1/2)/d lies at least 1/(2d) >= 2^-12 inside an integer interval and
+// the product's rounding error is below 2^-12.
+__device__ __forceinline__ int gen_div(int a, float rd) {
+  return (int)(((float)a + 0.5f) * rd);
+}
+
+// twiddles w^(k r), r = 1..R-1, of a stage's butterfly
+template <int R, int SIGN>
+__device__ __forceinline__ void gen_twiddle(float2* a, int k, int tstep,
+                                            const float2* __restrict__ tw) {
+#pragma unroll
+  for (int r = 1; r < R; ++r) {
+    float2 w = tw[k * r * tstep];
+    if (SIGN > 0) w.y = -w.y;
+    a[r] = cmul(a[r], w);
   }
 }
 
-// global index of element e of line l of block blk (axis 0: rows along p;
-// 1: columns along j at fixed i; 2: columns along i at fixed j)
-__device__ __forceinline__ long gen_index(int axis, int L, long blk, int l, int e) {
-  const long L2 = (long)L * L;
-  if (axis == 0) return (blk * kGenLines + l) * L + e;
-  const int ntile = L / kGenLines;
-  const long outer = blk / ntile;
-  const int p = (int)(blk % ntile) * kGenLines + l;
-  return axis == 1 ? outer * L2 + (long)e * L + p : (long)e * L2 + outer * L + p;
+// First stage (NS = 1): inputs from HBM (times W(|k| R) in mode 1), outputs
+// to the LDS line at j R + r.
+template <int R, int SIGN, int MODE>
+__device__ __forceinline__ void gen_first(const GenPlan& P, int jj, float2* line,
+                                          const float2* src, unsigned base, unsigned stride,
+                                          int msq, float2 kappa) {
+  const int L = P.L, Lr = L / R;
+#pragma unroll 2
+  for (int j = jj; j < Lr; j += 32) {
+    float2 a[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int e = j + r * Lr;
+      float2 v = src[base + (unsigned)e * stride];
+      if constexpr (MODE == 1) {
+        const float w = w_tophat_k(kappa, msq + fold_sq(e, L));
+        v = make_float2(v.x * w, v.y * w);
+      }
+      a[r] = v;
+    }
+    Dft<R, SIGN>::run(a);
+#pragma unroll
+    for (int r = 0; r < R; ++r) line[j * R + r] = a[r];
+  }
+  lds_pin();
+  __syncthreads();
 }
 
-template <int SIGN, int MODE>
+// Middle stage: all of a thread's butterflies are read before the barrier,
+// then written back into the same line.
+template <int R, int SIGN>
+__device__ __forceinline__ void gen_mid(const GenPlan& P, int st, int jj, float2* line,
+                                        const float2* __restrict__ tw) {
+  constexpr int MAXIT = (kGenMaxL / R + 31) / 32;
+  const int L = P.L, NS = P.ns[st], Lr = L / R, tstep = L / (NS * R);
+  const float rns = P.rns[st];
+  float2 a[MAXIT][R];
+#pragma unroll
+  for (int it = 0; it < MAXIT; ++it) {
+    const int j = jj + 32 * it;
+    if (j < Lr) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) a[it][r] = line[j + r * Lr];
+      gen_twiddle<R, SIGN>(a[it], j - gen_div(j, rns) * NS, tstep, tw);
+      Dft<R, SIGN>::run(a[it]);
+    }
+  }
+  lds_pin();
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < MAXIT; ++it) {
+    const int j = jj + 32 * it;
+    if (j < Lr) {
+      const int q = gen_div(j, rns), k = j - q * NS;
+      float2* y = line + q * NS * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) y[r * NS] = a[it][r];
+    }
+  }
+  lds_pin();
+  __syncthreads();
+}
+
+// Last stage (NS R = L, k = j): outputs at j + r L/R to HBM (mode 0), the
+// centre test (mode 2) or the unscaled pair (mode 3).
+template <int R, int SIGN, int MODE>
+__device__ __forceinline__ void gen_last(const GenPlan& P, int st, int jj, const float2* line,
+                                         float2* dst, unsigned base, unsigned stride,
+                                         const float2* __restrict__ tw, float sc,
+                                         int* __restrict__ wmap, int d2) {
+  const int L = P.L, NS = P.ns[st], Lr = L / R, tstep = L / (NS * R);
+#pragma unroll 2
+  for (int j = jj; j < Lr; j += 32) {
+    float2 a[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) a[r] = line[j + r * Lr];
+    gen_twiddle<R, SIGN>(a, j, tstep, tw);
+    Dft<R, SIGN>::run(a);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const unsigned idx = base + (unsigned)(j + r * Lr) * stride;
+      const float2 v = a[r];
+      if constexpr (MODE == 2) {
+        if (is_centre(v.x * sc, v.y)) wmap[idx] = d2;
+      } else if constexpr (MODE == 3) {
+        dst[idx] = make_float2(v.x, v.y / sc);
+      } else {
+        dst[idx] = v;
+      }
+    }
+  }
+}
+
+template <int SIGN, int MODE, bool ROWS>
 __global__ __launch_bounds__(256) void k_gen_pass(const float2* src, float2* dst, GenPlan plan,
                                                   int axis, const float2* __restrict__ tw,
                                                   float2 kappa, const float* __restrict__ bal,
                                                   int* __restrict__ wmap, int d2) {
   extern __shared__ float2 gsm[];
-  const int L = plan.L;
-  const long blk = blockIdx.x;
-  float2* in = gsm;
-  float2* out = gsm + kGenLines * L;
-  // load (contiguous along p for axis 0, across the 8 columns otherwise)
-  for (int t = threadIdx.x; t < kGenLines * L; t += blockDim.x) {
-    const int l = axis == 0 ? t / L : t % kGenLines;
-    const int e = axis == 0 ? t % L : t / kGenLines;
-    const long idx = gen_index(axis, L, blk, l, e);
-    float2 v = src[idx];
-    if constexpr (MODE == 1) {
-      const long L2 = (long)L * L;
-      const int m = fold_sq((int)(idx / L2), L) + fold_sq((int)((idx / L) % L), L) +
-                    fold_sq((int)(idx % L), L);
-      const float w = w_tophat_k(kappa, m);
-      v = make_float2(v.x * w, v.y * w);
-    }
-    in[l * L + e] = v;
+  const unsigned L = (unsigned)plan.L, L2 = L * L;
+  const int l = ROWS ? (int)(threadIdx.x >> 5) : (int)(threadIdx.x & 7);
+  const int jj = ROWS ? (int)(threadIdx.x & 31) : (int)(threadIdx.x >> 3);
+  unsigned base, stride;
+  int msq = 0;  // |k|^2 of the two coordinates fixed along the line
+  if (ROWS) {
+    const unsigned row = blockIdx.x * kGenLines + l;
+    base = row * L;
+    stride = 1;
+    if (MODE == 1) msq = fold_sq((int)(row / L), (int)L) + fold_sq((int)(row % L), (int)L);
+  } else {
+    const unsigned ntile = L / kGenLines;
+    const unsigned outer = blockIdx.x / ntile;
+    const unsigned p = (blockIdx.x % ntile) * kGenLines + l;
+    base = axis == 1 ? outer * L2 + p : outer * L + p;
+    stride = axis == 1 ? L : L2;
+    if (MODE == 1) msq = fold_sq((int)outer, (int)L) + fold_sq((int)p, (int)L);
   }
-  for (int st = 0; st < plan.nst; ++st) {
-    const int R = plan.radix[st], NS = plan.ns[st], Lr = L / R, tstep = L / (NS * R);
-    lds_pin();
-    __syncthreads();
-    for (int b = threadIdx.x; b < kGenLines * Lr; b += blockDim.x) {
-      const int l = b / Lr, j = b - l * Lr, k = j % NS;
-      const float2* x = in + l * L;
-      float2 a[5];
-#pragma unroll
-      for (int r = 0; r < 5; ++r) a[r] = r < R ? x[j + r * Lr] : make_float2(0.f, 0.f);
-      if (NS > 1) {
-#pragma unroll
-        for (int r = 1; r < 5; ++r)
-          if (r < R) {
-            float2 w = tw[k * r * tstep];
-            if (SIGN > 0) w.y = -w.y;
-            a[r] = cmul(a[r], w);
-          }
-      }
-      gen_dft<SIGN>(a, R);
-      float2* y = out + l * L + (j / NS) * NS * R + k;
-#pragma unroll
-      for (int r = 0; r < 5; ++r)
-        if (r < R) y[r * NS] = a[r];
-    }
-    float2* tmp = in;
-    in = out;
-    out = tmp;
+  const float sc = MODE >= 2 ? bal[0] : 1.0f;
+  float2* line = gsm + l * plan.lp;
+  switch (plan.radix[0]) {
+    case 8: gen_first<8, SIGN, MODE>(plan, jj, line, src, base, stride, msq, kappa); break;
+    case 4: gen_first<4, SIGN, MODE>(plan, jj, line, src, base, stride, msq, kappa); break;
+    case 2: gen_first<2, SIGN, MODE>(plan, jj, line, src, base, stride, msq, kappa); break;
+    case 3: gen_first<3, SIGN, MODE>(plan, jj, line, src, base, stride, msq, kappa); break;
+    default: gen_first<5, SIGN, MODE>(plan, jj, line, src, base, stride, msq, kappa); break;
   }
-  lds_pin();
-  __syncthreads();
-  const float sc = MODE >= 2 ? bal[0] : 0.0f;
-  for (int t = threadIdx.x; t < kGenLines * L; t += blockDim.x) {
-    const int l = axis == 0 ? t / L : t % kGenLines;
-    const int e = axis == 0 ? t % L : t / kGenLines;
-    const long idx = gen_index(axis, L, blk, l, e);
-    const float2 v = in[l * L + e];
-    if constexpr (MODE == 2) {
-      if (is_centre(v.x * sc, v.y)) wmap[idx] = d2;
-    } else if constexpr (MODE == 3) {
-      dst[idx] = make_float2(v.x, v.y / sc);
-    } else {
-      dst[idx] = v;
+  const int last = plan.nst - 1;
+  for (int st = 1; st < last; ++st) {
+    switch (plan.radix[st]) {
+      case 8: gen_mid<8, SIGN>(plan, st, jj, line, tw); break;
+      case 4: gen_mid<4, SIGN>(plan, st, jj, line, tw); break;
+      case 2: gen_mid<2, SIGN>(plan, st, jj, line, tw); break;
+      case 3: gen_mid<3, SIGN>(plan, st, jj, line, tw); break;
+      default: gen_mid<5, SIGN>(plan, st, jj, line, tw); break;
     }
   }
+#define GLS(R) gen_last<R, SIGN, MODE>(plan, last, jj, line, dst, base, stride, tw, sc, wmap, d2)
+  switch (plan.radix[last]) {
+    case 8: GLS(8); break;
+    case 4: GLS(4); break;
+    case 2: GLS(2); break;
+    case 3: GLS(3); break;
+    default: GLS(5); break;
+  }
+#undef GLS
 }
 
 __global__ void k_gen_pack(long n3, const float* __restrict__ a, const float* __restrict__ b,
@@ -969,11 +1049,12 @@ void launch_sum_partials(const double* partials, int np, double scale, double* o
 }
 
 bool gen_plan_make(int n, GenPlan* p) {
-  if (n < 8 || n % kGenLines != 0) return false;
+  if (n < 16 || n > kGenMaxL || n % kGenLines != 0) return false;
   int m = n, ns = 1, st = 0;
   while (m > 1) {
     int r;
-    if (m % 4 == 0) r = 4;
+    if (m % 8 == 0) r = 8;
+    else if (m % 4 == 0) r = 4;
     else if (m % 2 == 0) r = 2;
     else if (m % 3 == 0) r = 3;
     else if (m % 5 == 0) r = 5;
@@ -981,45 +1062,53 @@ bool gen_plan_make(int n, GenPlan* p) {
     if (st >= 16) return false;
     p->radix[st] = r;
     p->ns[st] = ns;
+    p->rns[st] = 1.0f / (float)ns;
     ns *= r;
     m /= r;
     ++st;
   }
   p->L = n;
   p->nst = st;
+  // line stride = 8 (mod 32) float2: the 8 lines of a column-pass wave
+  // start in different bank groups
+  p->lp = n + ((8 - n % 32) + 32) % 32;
   return true;
 }
 
-size_t gen_pass_lds(int L) { return sizeof(float2) * 2 * kGenLines * (size_t)L; }
+size_t gen_pass_lds(const GenPlan& p) { return sizeof(float2) * kGenLines * (size_t)p.lp; }
 
-bool gen_prepare(int L) {
-  // LDS of the transform blocks above 64 KiB (L > 512) is requested per kernel
-  const size_t lds = gen_pass_lds(L);
+#define GEN_KERNELS(X)                                                                     \
+  X(-1, 0, true) X(-1, 0, false) X(+1, 0, true) X(+1, 0, false) X(+1, 1, true)             \
+  X(+1, 1, false) X(+1, 2, true) X(+1, 2, false) X(+1, 3, true) X(+1, 3, false)
+
+bool gen_prepare(const GenPlan& p) {
+  // LDS of the transform blocks above 64 KiB is requested per kernel
+  const size_t lds = gen_pass_lds(p);
   if (lds <= 65536) return true;
   const int v = (int)lds;
   const hipFuncAttribute at = hipFuncAttributeMaxDynamicSharedMemorySize;
-  return hipFuncSetAttribute((const void*)k_gen_pass<-1, 0>, at, v) == hipSuccess &&
-         hipFuncSetAttribute((const void*)k_gen_pass<+1, 0>, at, v) == hipSuccess &&
-         hipFuncSetAttribute((const void*)k_gen_pass<+1, 1>, at, v) == hipSuccess &&
-         hipFuncSetAttribute((const void*)k_gen_pass<+1, 2>, at, v) == hipSuccess &&
-         hipFuncSetAttribute((const void*)k_gen_pass<+1, 3>, at, v) == hipSuccess;
+  bool ok = true;
+#define GA(SG, MD, RW) \
+  ok = ok && hipFuncSetAttribute((const void*)k_gen_pass<SG, MD, RW>, at, v) == hipSuccess;
+  GEN_KERNELS(GA)
+#undef GA
+  return ok;
 }
 
 void launch_gen_pass(const GenPlan& plan, int axis, int sign, int mode, const float2* src,
                      float2* dst, const float2* tw, float2 kappa, const float* bal, int* wmap,
                      int d2, hipStream_t s) {
-  const long L = plan.L;
-  const dim3 g((unsigned)(L * L / kGenLines)), b(256);
-  const size_t lds = gen_pass_lds(plan.L);
-#define GP(SG, MD)                                                                        \
-  hipLaunchKernelGGL((k_gen_pass<SG, MD>), g, b, lds, s, src, dst, plan, axis, tw, kappa, bal, \
-                     wmap, d2)
-  if (sign < 0) GP(-1, 0);
-  else if (mode == 1) GP(+1, 1);
-  else if (mode == 2) GP(+1, 2);
-  else if (mode == 3) GP(+1, 3);
-  else GP(+1, 0);
-#undef GP
+  const unsigned L = (unsigned)plan.L;
+  const dim3 g(L * L / kGenLines), b(256);
+  const size_t lds = gen_pass_lds(plan);
+  const bool rows = axis == 0;
+  const int sg = sign < 0 ? -1 : +1, md = sign < 0 ? 0 : mode;
+#define GL(SG, MD, RW)                                                                        \
+  if (sg == SG && md == MD && rows == RW)                                                     \
+    hipLaunchKernelGGL((k_gen_pass<SG, MD, RW>), g, b, lds, s, src, dst, plan, axis, tw, kappa, \
+                       bal, wmap, d2);
+  GEN_KERNELS(GL)
+#undef GL
 }
 
 void launch_gen_pack(long n3, const float* a, const float* b, const float* c, CellConsts k,

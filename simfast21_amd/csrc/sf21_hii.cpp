@@ -56,7 +56,7 @@ bool plan_size(int n) {
 // LDS, and dilation line passes whose lanes tile the side.
 bool generic_size(int n) {
   GenPlan g;
-  return n >= 16 && n <= 1024 && gen_plan_make(n, &g) && dt_q(n) > 0;
+  return n >= 16 && n <= 768 && gen_plan_make(n, &g) && dt_q(n) > 0;
 }
 
 bool size_supported(int n, int path) {
@@ -899,7 +899,7 @@ int sf21_hii_create(const sf21_hii_config* cfg, sf21_hii_ctx** out) {
   c->dx = c->L / c->n;
   c->dk = 2 * kPi / c->L;
   c->generic = cfg->path == SF21_PATH_FUSED && !plan_size(c->n) && c->n != 32;
-  if (c->generic && (!gen_plan_make(c->n, &c->gplan) || !gen_prepare(c->n))) {
+  if (c->generic && (!gen_plan_make(c->n, &c->gplan) || !gen_prepare(c->gplan))) {
     (void)hipGetLastError();
     delete c;
     return SF21_ERR_HIP;

@@ -107,7 +107,7 @@ dz = 1.0
 sim_length = 100
 hubble = 0.7
 N_halo = 128
-N_smoothed = 64
+N_smoothed = {n}
 fesc = 0.25
 Ion_cutoff = 0.999
 bubble_Rmax = 20
@@ -117,19 +117,21 @@ use_SFR = T
 
 
 @pytest.mark.gpu
-def test_cli_end_to_end_vs_oracle_cli(tmp_path, torch_cuda):
+@pytest.mark.parametrize("n", [64, 80])
+def test_cli_end_to_end_vs_oracle_cli(tmp_path, torch_cuda, n):
     """The drop-in CLI and the oracle CLI (FP64, exact dilation) on the same
-    base_dir: same files, same text tables to 1e-5, same boxes up to FP32."""
+    base_dir: same files, same text tables to 1e-5, same boxes up to FP32.
+    N_smoothed 80 has no register plan (LDS line transforms)."""
     dirs = {}
     for who in ("gpu", "oracle"):
         d = tmp_path / who
         (d / "delta").mkdir(parents=True)
         (d / "SFR").mkdir()
-        (d / "simfast21.ini").write_text(INI)
+        (d / "simfast21.ini").write_text(INI.format(n=n))
         for k, z in enumerate((7.0, 8.0, 9.0)):
-            c = sfr_case(n=64, z=z, seed=20 + k, zeta=1.0 + k)
-            c["delta"].astype(np.float32).tofile(d / "delta" / f"deltanl_z{z:.3f}_N64_L100.0.dat")
-            c["sfrd"].astype(np.float32).tofile(d / "SFR" / f"sfrd_z{z:.3f}_N64_L100.0.dat")
+            c = sfr_case(n=n, z=z, seed=20 + k, zeta=1.0 + k)
+            c["delta"].astype(np.float32).tofile(d / "delta" / f"deltanl_z{z:.3f}_N{n}_L100.0.dat")
+            c["sfrd"].astype(np.float32).tofile(d / "SFR" / f"sfrd_z{z:.3f}_N{n}_L100.0.dat")
         dirs[who] = d
     r = subprocess.run([str(sf.CLI_PATH), str(dirs["gpu"])], capture_output=True, text=True)
     assert r.returncode == 0, r.stdout + r.stderr
@@ -140,15 +142,15 @@ def test_cli_end_to_end_vs_oracle_cli(tmp_path, torch_cuda):
     zs = (dirs["gpu"] / "Output_text_files" / "zsim.txt").read_text().split()
     assert zs == (dirs["oracle"] / "Output_text_files" / "zsim.txt").read_text().split()
     assert len(zs) == 3
-    xg = np.array((dirs["gpu"] / "Output_text_files" / "x_HI_N64_L100.0.dat").read_text().split(), float)
-    xo = np.array((dirs["oracle"] / "Output_text_files" / "x_HI_N64_L100.0.dat").read_text().split(), float)
+    xg = np.array((dirs["gpu"] / "Output_text_files" / f"x_HI_N{n}_L100.0.dat").read_text().split(), float)
+    xo = np.array((dirs["oracle"] / "Output_text_files" / f"x_HI_N{n}_L100.0.dat").read_text().split(), float)
     assert xg.size == xo.size == 3
     # x_HI_*.dat lists z descending (get_HIIbubbles.c:522-527)
     for k, z in enumerate((9.0, 8.0, 7.0)):
-        f = f"xHII_z{z:.3f}_N64_L100.0.dat"
+        f = f"xHII_z{z:.3f}_N{n}_L100.0.dat"
         g = np.fromfile(dirs["gpu"] / "Ionization" / f, np.float32)
         o = np.fromfile(dirs["oracle"] / "Ionization" / f, np.float32)
-        assert g.size == o.size == 64 ** 3
+        assert g.size == o.size == n ** 3
         # within 1e-5, or attributed to the cells whose value differs: at 64^3
         # one FP32-undecidable small-radius centre (its per-radius flips are
         # checked against FP64 in test_fused_vs_oracle) moves the mean by
@@ -192,7 +194,7 @@ def test_halo_emissivity_vs_oracle(torch_cuda):
 
 @pytest.mark.gpu
 def test_cli_halo_mode_vs_oracle_cli(tmp_path, torch_cuda):
-    ini = INI.replace("use_SFR = T", "use_SFR = F")
+    ini = INI.format(n=64).replace("use_SFR = T", "use_SFR = F")
     dirs = {}
     for who in ("gpu", "oracle"):
         d = tmp_path / who

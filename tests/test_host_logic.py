@@ -117,13 +117,12 @@ def test_create_without_gpu_fails_cleanly():
 
 @pytest.mark.parametrize("n,path,ok", [
     (600, "FUSED", True), (80, "FUSED", True), (120, "FUSED", True), (200, "FUSED", True),
-    (360, "FUSED", True), (1024, "FUSED", True), (600, "PLAIN", False), (56, "FUSED", False),
-    (100, "FUSED", False), (1040, "FUSED", False), (1000, "FUSED", False), (8, "FUSED", False)])
+    (360, "FUSED", True), (768, "FUSED", True), (600, "PLAIN", False), (56, "FUSED", False),
+    (100, "FUSED", False), (1024, "FUSED", False), (1000, "FUSED", False), (8, "FUSED", False)])
 def test_size_acceptance(n, path, ok):
     """Sides the fused finder takes: the register-plan sides and any other
-    multiple of 8 from 16 to 1024 with prime factors 2, 3 and 5 whose
-    dilation lanes tile it (1000 = 8 * 125 does not: no q <= 16 with n/q a
-    multiple of 8 and at most 64).  The plain path keeps the plan sides.
+    multiple of 8 from 16 to 768 with prime factors 2, 3 and 5 whose
+    dilation lanes tile it.  The plain path keeps the plan sides.
     Without a GPU an accepted size fails later, at device selection."""
     if _have_gpu():
         pytest.skip("a HIP device is present")
