@@ -176,6 +176,9 @@ void launch_to_complex(long n3, const float* in, float2* out, hipStream_t s);
 // ---- 21-cm brightness temperature ----
 int t21_partials_count(long n3);
 void launch_vgrad_filter(int n, float2* spec, int axis, hipStream_t s);
+// the same on the full n^3 spectrum of a complex transform (generic sides)
+void launch_vgrad_filter_full(int n, float2* spec, int axis, hipStream_t s);
+void launch_real_part(long n3, const float2* in, float* out, hipStream_t s);
 void launch_t21_cells(long n3, const float* delta, const float* xhii, const float* dv_raw,
                       const float* xc, const float* xa, const float* tk, float vfac, float c21,
                       float tcmb, float* out, double* part, double* sums, hipStream_t s);

@@ -34,6 +34,30 @@ __global__ void k_vgrad_filter(int n, float2* __restrict__ spec, int axis) {
   }
 }
 
+// Full-spectrum form for box sides without a register plan (complex
+// transform of the real box): the same factor with the p index folded like
+// i and j.  The spectrum stays Hermitian, so the inverse is real.
+__global__ void k_vgrad_filter_full(int n, float2* __restrict__ spec, int axis) {
+  const long n3 = (long)n * n * n;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n3;
+       i += (long)gridDim.x * blockDim.x) {
+    const int p = (int)(i % n);
+    const long r = i / n;
+    const int a2 = fold_sq((int)(r / n), n), j2 = fold_sq((int)(r % n), n), q2 = fold_sq(p, n);
+    const int m = a2 + j2 + q2;
+    const int num = axis == 1 ? a2 : (axis == 2 ? j2 : q2);
+    const float f = m == 0 ? 0.0f : (float)num / (float)m;
+    const float2 v = spec[i];
+    spec[i] = make_float2(v.x * f, v.y * f);
+  }
+}
+
+__global__ void k_real_part(long n3, const float2* __restrict__ in, float* __restrict__ out) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n3;
+       i += (long)gridDim.x * blockDim.x)
+    out[i] = in[i].x;
+}
+
 // Per cell: spin factor x/(1+x) (1 - Tcmb/TK), x = x_alpha + x_c (t21.c:121-154;
 // 1 without the spin boxes), dv = clamp(vfac * dv_raw, -0.5, 1) (:196-206),
 // t21 = c21 (1 + delta) spin/(1 + dv) (:208-212) times (1 - xHII) (:215-223).
@@ -110,6 +134,15 @@ int t21_partials_count(long n3) { return (int)blocks_t21(n3); }
 void launch_vgrad_filter(int n, float2* spec, int axis, hipStream_t s) {
   const long nh = (long)n * n * (n / 2 + 1);
   hipLaunchKernelGGL(k_vgrad_filter, dim3(blocks_t21(nh)), dim3(256), 0, s, n, spec, axis);
+}
+
+void launch_vgrad_filter_full(int n, float2* spec, int axis, hipStream_t s) {
+  const long n3 = (long)n * n * n;
+  hipLaunchKernelGGL(k_vgrad_filter_full, dim3(blocks_t21(n3)), dim3(256), 0, s, n, spec, axis);
+}
+
+void launch_real_part(long n3, const float2* in, float* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_real_part, dim3(blocks_t21(n3)), dim3(256), 0, s, n3, in, out);
 }
 
 void launch_t21_cells(long n3, const float* delta, const float* xhii, const float* dv_raw,

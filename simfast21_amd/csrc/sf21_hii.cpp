@@ -862,7 +862,7 @@ int create_deposit(const sf21_hii_config* cfg, sf21_hii_ctx** out) {
 // Entry points that need the transforms and the bubble finder's buffers.
 bool is_deposit(const sf21_hii_ctx* c) { return c && c->cfg.path == SF21_PATH_DEPOSIT; }
 // Entry points built on the register transform plans (autotuner, power
-// spectrum, 21-cm step, differentiable mode).
+// spectrum, differentiable mode).
 bool is_generic(const sf21_hii_ctx* c) { return c && c->generic; }
 
 }  // namespace
@@ -1535,7 +1535,7 @@ int sf21_hii_run_slice_halo(sf21_hii_ctx* c, const float* delta, const float* ha
 int sf21_t21_dev(sf21_hii_ctx* c, const sf21_params* p, double z, const float* delta,
                  const float* xhii, const float* xc, const float* xalpha, const float* tk,
                  float* out, double* mean_t21, double* mean_ts) {
-  if (is_deposit(c) || is_generic(c)) return SF21_ERR_ARG;
+  if (is_deposit(c)) return SF21_ERR_ARG;
   if (!c || !p || !delta || !xhii || !out || c->n < 64) return SF21_ERR_ARG;
   const bool spin = xc || xalpha || tk;
   if (spin && !(xc && xalpha && tk)) return SF21_ERR_ARG;
@@ -1556,9 +1556,26 @@ int sf21_t21_dev(sf21_hii_ctx* c, const sf21_params* p, double z, const float* d
   // -dD/dt / (H D) times the unnormalised transform's 1/N^3 (dx^3/L^3)
   const double vfac = -dgdt / hz / growth / (double)n3;
   const double tcmb = 2.725 * (1. + z);
-  r2c3d(c, delta, c->spec);
-  launch(c, "vgrad_filter", [&] { launch_vgrad_filter(n, c->spec, p->vel_comp, c->stream); });
-  c2r3d(c, c->spec, c->t21_dv);
+  if (c->generic) {
+    // complex transforms of the real box through the generic line passes
+    hipStream_t s = c->stream;
+    const float2 none = make_float2(0.f, 0.f);
+    launch(c, "to_complex", [&] { launch_to_complex(n3, delta, c->spec, s); });
+    for (int axis = 0; axis < 3; ++axis)
+      launch(c, "gen_fwd", [&] {
+        launch_gen_pass(c->gplan, axis, -1, 0, c->spec, c->spec, c->tw, none, nullptr, nullptr, 0, s);
+      });
+    launch(c, "vgrad_filter", [&] { launch_vgrad_filter_full(n, c->spec, p->vel_comp, s); });
+    for (int axis = 0; axis < 3; ++axis)
+      launch(c, "gen_inv", [&] {
+        launch_gen_pass(c->gplan, axis, +1, 0, c->spec, c->spec, c->tw, none, nullptr, nullptr, 0, s);
+      });
+    launch(c, "real_part", [&] { launch_real_part(n3, c->spec, c->t21_dv, s); });
+  } else {
+    r2c3d(c, delta, c->spec);
+    launch(c, "vgrad_filter", [&] { launch_vgrad_filter(n, c->spec, p->vel_comp, c->stream); });
+    c2r3d(c, c->spec, c->t21_dv);
+  }
   double* sums = c->t21_part + 2 * t21_partials_count(n3);
   launch(c, "t21_cells", [&] {
     launch_t21_cells(n3, delta, xhii, c->t21_dv, xc, xalpha, tk, (float)vfac, (float)c21,
