@@ -700,8 +700,13 @@ __device__ __forceinline__ void gen_twiddle(float2* a, int k, int tstep,
   }
 }
 
-// First stage (NS = 1): inputs from HBM (times W(|k| R) in mode 1), outputs
-// to the LDS line at j R + r.
+// First stage (radix 8, NS = 1): inputs from HBM (times W(|k| R) in mode 1).
+// Output position o = 8 j + r is kept transposed, at slot r L/8 + j, so that
+// the stores of consecutive butterflies are consecutive (a stride-8 store
+// pattern conflicts in the LDS banks); the next stage reads position o at
+// gen_t1_slot(o).
+__device__ __forceinline__ int gen_t1_slot(int o, int lr0) { return (o & 7) * lr0 + (o >> 3); }
+
 template <int R, int SIGN, int MODE>
 __device__ __forceinline__ void gen_first(const GenPlan& P, int jj, float2* line,
                                           const float2* src, unsigned base, unsigned stride,
@@ -722,15 +727,16 @@ __device__ __forceinline__ void gen_first(const GenPlan& P, int jj, float2* line
     }
     Dft<R, SIGN>::run(a);
 #pragma unroll
-    for (int r = 0; r < R; ++r) line[j * R + r] = a[r];
+    for (int r = 0; r < R; ++r) line[r * Lr + j] = a[r];
   }
   lds_pin();
   __syncthreads();
 }
 
 // Middle stage: all of a thread's butterflies are read before the barrier,
-// then written back into the same line.
-template <int R, int SIGN>
+// then written back into the same line.  T1: the stage after the first
+// (inputs in the transposed slots).
+template <int R, int SIGN, bool T1>
 __device__ __forceinline__ void gen_mid(const GenPlan& P, int st, int jj, float2* line,
                                         const float2* __restrict__ tw) {
   constexpr int MAXIT = (kGenMaxL / R + 31) / 32;
@@ -742,7 +748,10 @@ __device__ __forceinline__ void gen_mid(const GenPlan& P, int st, int jj, float2
     const int j = jj + 32 * it;
     if (j < Lr) {
 #pragma unroll
-      for (int r = 0; r < R; ++r) a[it][r] = line[j + r * Lr];
+      for (int r = 0; r < R; ++r) {
+        const int o = j + r * Lr;
+        a[it][r] = line[T1 ? gen_t1_slot(o, L >> 3) : o];
+      }
       gen_twiddle<R, SIGN>(a[it], j - gen_div(j, rns) * NS, tstep, tw);
       Dft<R, SIGN>::run(a[it]);
     }
@@ -765,7 +774,7 @@ __device__ __forceinline__ void gen_mid(const GenPlan& P, int st, int jj, float2
 
 // Last stage (NS R = L, k = j): outputs at j + r L/R to HBM (mode 0), the
 // centre test (mode 2) or the unscaled pair (mode 3).
-template <int R, int SIGN, int MODE>
+template <int R, int SIGN, int MODE, bool T1>
 __device__ __forceinline__ void gen_last(const GenPlan& P, int st, int jj, const float2* line,
                                          float2* dst, unsigned base, unsigned stride,
                                          const float2* __restrict__ tw, float sc,
@@ -775,7 +784,10 @@ __device__ __forceinline__ void gen_last(const GenPlan& P, int st, int jj, const
   for (int j = jj; j < Lr; j += 32) {
     float2 a[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) a[r] = line[j + r * Lr];
+    for (int r = 0; r < R; ++r) {
+      const int o = j + r * Lr;
+      a[r] = line[T1 ? gen_t1_slot(o, L >> 3) : o];
+    }
     gen_twiddle<R, SIGN>(a, j, tstep, tw);
     Dft<R, SIGN>::run(a);
 #pragma unroll
@@ -793,57 +805,78 @@ __device__ __forceinline__ void gen_last(const GenPlan& P, int st, int jj, const
   }
 }
 
+// 4 waves per SIMD (128 VGPRs): the LDS line buffers allow 4 blocks per CU
+// at 600, and the bound (a few spilled registers in the column variants) ran
+// 5% faster than 130 VGPRs at 3 waves
 template <int SIGN, int MODE, bool ROWS>
-__global__ __launch_bounds__(256) void k_gen_pass(const float2* src, float2* dst, GenPlan plan,
+__global__ __launch_bounds__(256, 4) void k_gen_pass(const float2* src, float2* dst, GenPlan plan,
                                                   int axis, const float2* __restrict__ tw,
                                                   float2 kappa, const float* __restrict__ bal,
                                                   int* __restrict__ wmap, int d2) {
   extern __shared__ float2 gsm[];
-  const unsigned L = (unsigned)plan.L, L2 = L * L;
+  const unsigned L = (unsigned)plan.L, L2 = L * L, blk = blockIdx.x;
   const int l = ROWS ? (int)(threadIdx.x >> 5) : (int)(threadIdx.x & 7);
   const int jj = ROWS ? (int)(threadIdx.x & 31) : (int)(threadIdx.x >> 3);
   unsigned base, stride;
   int msq = 0;  // |k|^2 of the two coordinates fixed along the line
   if (ROWS) {
-    const unsigned row = blockIdx.x * kGenLines + l;
+    const unsigned row = blk * kGenLines + l;
     base = row * L;
     stride = 1;
     if (MODE == 1) msq = fold_sq((int)(row / L), (int)L) + fold_sq((int)(row % L), (int)L);
   } else {
     const unsigned ntile = L / kGenLines;
-    const unsigned outer = blockIdx.x / ntile;
-    const unsigned p = (blockIdx.x % ntile) * kGenLines + l;
+    const unsigned outer = blk / ntile;
+    const unsigned p = (blk % ntile) * kGenLines + l;
     base = axis == 1 ? outer * L2 + p : outer * L + p;
     stride = axis == 1 ? L : L2;
     if (MODE == 1) msq = fold_sq((int)outer, (int)L) + fold_sq((int)p, (int)L);
   }
   const float sc = MODE >= 2 ? bal[0] : 1.0f;
   float2* line = gsm + l * plan.lp;
-  switch (plan.radix[0]) {
-    case 8: gen_first<8, SIGN, MODE>(plan, jj, line, src, base, stride, msq, kappa); break;
-    case 4: gen_first<4, SIGN, MODE>(plan, jj, line, src, base, stride, msq, kappa); break;
-    case 2: gen_first<2, SIGN, MODE>(plan, jj, line, src, base, stride, msq, kappa); break;
-    case 3: gen_first<3, SIGN, MODE>(plan, jj, line, src, base, stride, msq, kappa); break;
-    default: gen_first<5, SIGN, MODE>(plan, jj, line, src, base, stride, msq, kappa); break;
-  }
+  // the first radix is 8 on every generic side (multiples of 8)
+  gen_first<8, SIGN, MODE>(plan, jj, line, src, base, stride, msq, kappa);
   const int last = plan.nst - 1;
-  for (int st = 1; st < last; ++st) {
-    switch (plan.radix[st]) {
-      case 8: gen_mid<8, SIGN>(plan, st, jj, line, tw); break;
-      case 4: gen_mid<4, SIGN>(plan, st, jj, line, tw); break;
-      case 2: gen_mid<2, SIGN>(plan, st, jj, line, tw); break;
-      case 3: gen_mid<3, SIGN>(plan, st, jj, line, tw); break;
-      default: gen_mid<5, SIGN>(plan, st, jj, line, tw); break;
+#define GMS(R, T) gen_mid<R, SIGN, T>(plan, st, jj, line, tw)
+#define GLS(R, T) \
+  gen_last<R, SIGN, MODE, T>(plan, last, jj, line, dst, base, stride, tw, sc, wmap, d2)
+  if (last == 1) {
+    switch (plan.radix[1]) {
+      case 8: GLS(8, true); break;
+      case 4: GLS(4, true); break;
+      case 2: GLS(2, true); break;
+      case 3: GLS(3, true); break;
+      default: GLS(5, true); break;
+    }
+    return;
+  }
+  {
+    const int st = 1;
+    switch (plan.radix[1]) {
+      case 8: GMS(8, true); break;
+      case 4: GMS(4, true); break;
+      case 2: GMS(2, true); break;
+      case 3: GMS(3, true); break;
+      default: GMS(5, true); break;
     }
   }
-#define GLS(R) gen_last<R, SIGN, MODE>(plan, last, jj, line, dst, base, stride, tw, sc, wmap, d2)
-  switch (plan.radix[last]) {
-    case 8: GLS(8); break;
-    case 4: GLS(4); break;
-    case 2: GLS(2); break;
-    case 3: GLS(3); break;
-    default: GLS(5); break;
+  for (int st = 2; st < last; ++st) {
+    switch (plan.radix[st]) {
+      case 8: GMS(8, false); break;
+      case 4: GMS(4, false); break;
+      case 2: GMS(2, false); break;
+      case 3: GMS(3, false); break;
+      default: GMS(5, false); break;
+    }
   }
+  switch (plan.radix[last]) {
+    case 8: GLS(8, false); break;
+    case 4: GLS(4, false); break;
+    case 2: GLS(2, false); break;
+    case 3: GLS(3, false); break;
+    default: GLS(5, false); break;
+  }
+#undef GMS
 #undef GLS
 }
 
@@ -1069,6 +1102,7 @@ bool gen_plan_make(int n, GenPlan* p) {
   }
   p->L = n;
   p->nst = st;
+  if (p->radix[0] != 8 || st < 2) return false;  // the kernels' first stage is radix 8
   // line stride = 8 (mod 32) float2: the 8 lines of a column-pass wave
   // start in different bank groups
   p->lp = n + ((8 - n % 32) + 32) % 32;
