@@ -64,7 +64,7 @@ typedef struct sf21_hii_config {
                             path also any other multiple of 8 in 16..768 with factors
                             2, 3, 5 and n/q a multiple of 8, at most 64, for some q <= 16
                             (e.g. 80, 120, 200, 360, 600); those sides refuse the
-                            autotuner, power spectrum and differentiable mode */
+                            autotuner and the differentiable mode */
   double box_len;        /* box side L (unit of the radii) */
   int n_radii;           /* ladder length */
   const double *radii;   /* descending ladder (get_HIIbubbles.c:298-461); radii >= 2*dx are

@@ -109,8 +109,10 @@ void launch_clamp_sum(long n3, float* bubblef, const float* fresid, double* part
 
 // ---- binned power spectrum ----
 int pk_partials_count(int n);
+// full: spec is the n^3 spectrum of a complex transform (its p <= n/2 half binned)
 void launch_power_spectrum(int n, const float2* spec, const short* bin_of_m, const float* kabs_of_m,
-                           int nbins, double norm, double* part, double* out, hipStream_t s);
+                           int nbins, double norm, double* part, double* out, hipStream_t s,
+                           bool full = false);
 
 // ---- differentiable mode (exact forward, tau-relaxed adjoint) ----
 // slots of the FP64 device state array

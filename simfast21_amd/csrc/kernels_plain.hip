@@ -370,22 +370,26 @@ namespace sf21 {
 // ------------------------------------------------------------------------
 // Binned power spectrum of a half spectrum (tools/power3d.c:172-191
 // normalisation): per block FP64 sums of |F|^2, |k| and counts per bin in
-// LDS, written as partials and reduced in a fixed order.
+// LDS, written as partials and reduced in a fixed order.  full: the n^3
+// spectrum of a complex transform of the real box (generic sides); only its
+// half p <= n/2 is binned, the same modes as the half spectrum's.
 // ------------------------------------------------------------------------
 constexpr int kPkMaxBins = 64;
 
 __global__ __launch_bounds__(256) void k_pk_partial(int n, const float2* __restrict__ spec,
                                                     const short* __restrict__ bin_of_m,
                                                     const float* __restrict__ kabs_of_m,
-                                                    int nbins, double* __restrict__ part) {
+                                                    int nbins, double* __restrict__ part,
+                                                    bool full) {
   __shared__ double s_p[kPkMaxBins], s_k[kPkMaxBins], s_c[kPkMaxBins];
   for (int b = threadIdx.x; b < nbins; b += blockDim.x) s_p[b] = s_k[b] = s_c[b] = 0.0;
   __syncthreads();
-  const int h = n / 2 + 1;
+  const int h = full ? n : n / 2 + 1;
   const long nh = (long)n * n * h;
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < nh;
        i += (long)gridDim.x * blockDim.x) {
     const int q = (int)(i % h);
+    if (q > n / 2) continue;
     const long r = i / h;
     const int m = fold_sq((int)(r / n), n) + fold_sq((int)(r % n), n) + q * q;
     const int b = bin_of_m[m];
@@ -425,10 +429,11 @@ int pk_partials_count(int n) {
 }
 
 void launch_power_spectrum(int n, const float2* spec, const short* bin_of_m, const float* kabs_of_m,
-                           int nbins, double norm, double* part, double* out, hipStream_t s) {
+                           int nbins, double norm, double* part, double* out, hipStream_t s,
+                           bool full) {
   const int nb = pk_partials_count(n);
   hipLaunchKernelGGL(k_pk_partial, dim3(nb), dim3(256), 0, s, n, spec, bin_of_m, kabs_of_m, nbins,
-                     part);
+                     part, full);
   hipLaunchKernelGGL(k_pk_final, dim3(1), dim3(kPkMaxBins), 0, s, part, nb, nbins, norm, out);
 }
 

@@ -520,13 +520,24 @@ int ensure_pk(sf21_hii_ctx* c, int nbins) {
 // P(k) of a real box (tools/power3d.c:172-191 normalisation, dl^3/N) into
 // c->pk_out: [0, nbins) power, [nbins, 2 nbins) mean |k|, then counts.
 // half: scratch of n*n*(n/2+1) complex values, left holding the spectrum.
+// half: the half spectrum, or on generic sides the full n^3 spectrum
 void enqueue_pk(sf21_hii_ctx* c, const float* field, float2* half, int nbins) {
-  r2c3d(c, field, half);
+  hipStream_t s = c->stream;
+  if (c->generic) {
+    const float2 none = make_float2(0.f, 0.f);
+    launch(c, "to_complex", [&] { launch_to_complex(c->n3, field, half, s); });
+    for (int axis = 0; axis < 3; ++axis)
+      launch(c, "gen_fwd", [&] {
+        launch_gen_pass(c->gplan, axis, -1, 0, half, half, c->tw, none, nullptr, nullptr, 0, s);
+      });
+  } else {
+    r2c3d(c, field, half);
+  }
   const double dl = c->dx;
   const double norm = dl * dl * dl / ((double)c->n3);
   launch(c, "power_spectrum", [&] {
     launch_power_spectrum(c->n, half, c->pk_bin, c->pk_kabs, nbins, norm, c->pk_part, c->pk_out,
-                          c->stream);
+                          s, c->generic);
   });
 }
 
@@ -861,8 +872,8 @@ int create_deposit(const sf21_hii_config* cfg, sf21_hii_ctx** out) {
 
 // Entry points that need the transforms and the bubble finder's buffers.
 bool is_deposit(const sf21_hii_ctx* c) { return c && c->cfg.path == SF21_PATH_DEPOSIT; }
-// Entry points built on the register transform plans (autotuner, power
-// spectrum, differentiable mode).
+// Entry points built on the register transform plans (autotuner,
+// differentiable mode).
 bool is_generic(const sf21_hii_ctx* c) { return c && c->generic; }
 
 }  // namespace
@@ -1451,13 +1462,13 @@ int sf21_hii_reset_stats(sf21_hii_ctx* c) {
 // ---- binned power spectrum ------------------------------------------------
 int sf21_hii_power_spectrum_dev(sf21_hii_ctx* c, const float* field, int nbins, double* pk,
                                 double* kmean, double* counts) {
-  if (is_deposit(c) || is_generic(c)) return SF21_ERR_ARG;
+  if (is_deposit(c)) return SF21_ERR_ARG;
   if (!c || !field || nbins < 1 || nbins > 64 || c->n < 64) return SF21_ERR_ARG;
   if (hipSetDevice(c->cfg.device) != hipSuccess) return SF21_ERR_HIP;
   (void)hipGetLastError();
   int rc;
   if ((rc = ensure_pk(c, nbins))) return rc;
-  // the half spectrum goes to the context's spectrum buffer (n^3 >= n*n*(n/2+1))
+  // the spectrum goes to the context's spectrum buffer (n^3 >= n*n*(n/2+1))
   enqueue_pk(c, field, c->spec, nbins);
   std::vector<double> host(3 * nbins);
   if (hipMemcpyAsync(host.data(), c->pk_out, sizeof(double) * 3 * nbins, hipMemcpyDeviceToHost,

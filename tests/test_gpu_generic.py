@@ -104,8 +104,8 @@ def test_generic_600_dilation_and_mean(torch_cuda):
 
 @pytest.mark.gpu
 def test_generic_refusals(torch_cuda):
-    """Entry points built on the register plans refuse generic sides with
-    an argument error instead of running."""
+    """Entry points built on the register plans (autotuner, differentiable
+    mode) refuse generic sides with an argument error instead of running."""
     torch = torch_cuda
     n = 80
     case = direct_case(n, 1.0)
@@ -114,8 +114,6 @@ def test_generic_refusals(torch_cuda):
     b = torch.from_numpy(case["ion"].ravel()).cuda()
     with pytest.raises(sf.Sf21Error):
         ctx.autotune(a, b)
-    with pytest.raises(sf.Sf21Error):
-        ctx.power_spectrum(a)
     with pytest.raises(sf.Sf21Error):
         ctx.diff_forward(a, b, 1.0)
     x, neu = ctx.run_slice(a, b)  # the context still runs

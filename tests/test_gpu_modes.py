@@ -222,10 +222,11 @@ def test_cli_halo_mode_vs_oracle_cli(tmp_path, torch_cuda):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("n", [64, 128])
+@pytest.mark.parametrize("n", [64, 128, 80, 120])
 def test_power_spectrum_vs_oracle(torch_cuda, n):
     """20-bin P(k) (tools/power3d.c normalisation) against the oracle's FP64
-    restatement and numpy's rfftn."""
+    restatement and numpy's rfftn.  80 and 120 have no register plan: a
+    complex transform of the real box, its p <= n/2 half binned."""
     torch = torch_cuda
     side = 100.0
     g = synth.gaussian_field(n, -2.5, 1.2, seed=9).astype(np.float64)
