@@ -119,3 +119,27 @@ def test_generic_refusals(torch_cuda):
     x, neu = ctx.run_slice(a, b)  # the context still runs
     assert 0.0 <= neu <= 1.0
     ctx.close()
+
+
+@pytest.mark.gpu
+def test_generic_batch_graph_equals_single_runs(torch_cuda):
+    """Several slices of a generic side in one hipGraph (sf21_hii_run_batch_dev):
+    each equals its single run bit for bit, on the first launch and on
+    replays."""
+    torch = torch_cuda
+    n = 80
+    cases = [direct_case(n, zeta=0.6 + 0.5 * k, seed=7100 + k) for k in range(3)]
+    ctx = sf.HiiContext(n, cases[0]["side"], cases[0]["radii"], sf.INPUT_DIRECT, sf.PATH_FUSED)
+    a = [torch.from_numpy(c["rec"].ravel()).cuda() for c in cases]
+    b = [torch.from_numpy(c["ion"].ravel()).cuda() for c in cases]
+    single = []
+    for k in range(3):
+        x, neu = ctx.run_slice(a[k], b[k])
+        single.append((x.cpu().numpy(), neu))
+    outs = [torch.empty_like(x) for x in a]
+    for rep in range(3):
+        _, neus = ctx.run_batch(a, b, out_list=outs)
+        for k in range(3):
+            assert np.array_equal(outs[k].cpu().numpy(), single[k][0]), (rep, k)
+            assert neus[k] == single[k][1], (rep, k)
+    ctx.close()
