@@ -707,11 +707,13 @@ __device__ __forceinline__ void gen_twiddle(float2* a, int k, int tstep,
 // gen_t1_slot(o).
 __device__ __forceinline__ int gen_t1_slot(int o, int lr0) { return (o & 7) * lr0 + (o >> 3); }
 
-template <int R, int SIGN, int MODE>
+// LC (and NSC): the side (and the stage span) as compile-time constants for
+// the specialised sides, 0 for the runtime plan.
+template <int R, int SIGN, int MODE, int LC>
 __device__ __forceinline__ void gen_first(const GenPlan& P, int jj, float2* line,
                                           const float2* src, unsigned base, unsigned stride,
                                           int msq, float2 kappa) {
-  const int L = P.L, Lr = L / R;
+  const int L = LC ? LC : P.L, Lr = L / R;
 #pragma unroll 2
   for (int j = jj; j < Lr; j += 32) {
     float2 a[R];
@@ -736,12 +738,13 @@ __device__ __forceinline__ void gen_first(const GenPlan& P, int jj, float2* line
 // Middle stage: all of a thread's butterflies are read before the barrier,
 // then written back into the same line.  T1: the stage after the first
 // (inputs in the transposed slots).
-template <int R, int SIGN, bool T1>
+template <int R, int SIGN, bool T1, int LC, int NSC>
 __device__ __forceinline__ void gen_mid(const GenPlan& P, int st, int jj, float2* line,
                                         const float2* __restrict__ tw) {
-  constexpr int MAXIT = (kGenMaxL / R + 31) / 32;
-  const int L = P.L, NS = P.ns[st], Lr = L / R, tstep = L / (NS * R);
+  constexpr int MAXIT = ((LC ? LC : kGenMaxL) / R + 31) / 32;
+  const int L = LC ? LC : P.L, NS = LC ? NSC : P.ns[st], Lr = L / R, tstep = L / (NS * R);
   const float rns = P.rns[st];
+  auto quot = [&](int j) { return LC ? (int)((unsigned)j / (unsigned)NSC) : gen_div(j, rns); };
   float2 a[MAXIT][R];
 #pragma unroll
   for (int it = 0; it < MAXIT; ++it) {
@@ -752,7 +755,7 @@ __device__ __forceinline__ void gen_mid(const GenPlan& P, int st, int jj, float2
         const int o = j + r * Lr;
         a[it][r] = line[T1 ? gen_t1_slot(o, L >> 3) : o];
       }
-      gen_twiddle<R, SIGN>(a[it], j - gen_div(j, rns) * NS, tstep, tw);
+      gen_twiddle<R, SIGN>(a[it], j - quot(j) * NS, tstep, tw);
       Dft<R, SIGN>::run(a[it]);
     }
   }
@@ -762,7 +765,7 @@ __device__ __forceinline__ void gen_mid(const GenPlan& P, int st, int jj, float2
   for (int it = 0; it < MAXIT; ++it) {
     const int j = jj + 32 * it;
     if (j < Lr) {
-      const int q = gen_div(j, rns), k = j - q * NS;
+      const int q = quot(j), k = j - q * NS;
       float2* y = line + q * NS * R + k;
 #pragma unroll
       for (int r = 0; r < R; ++r) y[r * NS] = a[it][r];
@@ -774,12 +777,12 @@ __device__ __forceinline__ void gen_mid(const GenPlan& P, int st, int jj, float2
 
 // Last stage (NS R = L, k = j): outputs at j + r L/R to HBM (mode 0), the
 // centre test (mode 2) or the unscaled pair (mode 3).
-template <int R, int SIGN, int MODE, bool T1>
+template <int R, int SIGN, int MODE, bool T1, int LC, int NSC>
 __device__ __forceinline__ void gen_last(const GenPlan& P, int st, int jj, const float2* line,
                                          float2* dst, unsigned base, unsigned stride,
                                          const float2* __restrict__ tw, float sc,
                                          int* __restrict__ wmap, int d2) {
-  const int L = P.L, NS = P.ns[st], Lr = L / R, tstep = L / (NS * R);
+  const int L = LC ? LC : P.L, NS = LC ? NSC : P.ns[st], Lr = L / R, tstep = L / (NS * R);
 #pragma unroll 2
   for (int j = jj; j < Lr; j += 32) {
     float2 a[R];
@@ -808,13 +811,23 @@ __device__ __forceinline__ void gen_last(const GenPlan& P, int st, int jj, const
 // 4 waves per SIMD (128 VGPRs): the LDS line buffers allow 4 blocks per CU
 // at 600, and the bound (a few spilled registers in the column variants) ran
 // 5% faster than 130 VGPRs at 3 waves
-template <int SIGN, int MODE, bool ROWS>
+// LC = 600: the sample configuration's side with its plan 8, 3, 5, 5 (spans
+// 1, 8, 24, 120) and LDS line stride 616 as constants; 0: the runtime plan.
+template <int SIGN, int MODE, bool ROWS, int LC>
 __global__ __launch_bounds__(256, 4) void k_gen_pass(const float2* src, float2* dst, GenPlan plan,
                                                   int axis, const float2* __restrict__ tw,
                                                   float2 kappa, const float* __restrict__ bal,
                                                   int* __restrict__ wmap, int d2) {
   extern __shared__ float2 gsm[];
-  const unsigned L = (unsigned)plan.L, L2 = L * L, blk = blockIdx.x;
+  const unsigned L = LC ? (unsigned)LC : (unsigned)plan.L, L2 = L * L;
+  // i-axis passes: consecutive tiles (which share 128 B lines) on one XCD --
+  // workgroups are dealt round-robin over the 8 XCDs, and the grid (L^2/8
+  // blocks, L a multiple of 8) is a multiple of 8.  Measured at 600: the
+  // filtered i pass 1.30 -> 1.15 ms; the j pass 0.80 -> 0.81 ms, so it keeps
+  // the plain order.
+  const unsigned blk = (!ROWS && axis == 2)
+                           ? (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3)
+                           : blockIdx.x;
   const int l = ROWS ? (int)(threadIdx.x >> 5) : (int)(threadIdx.x & 7);
   const int jj = ROWS ? (int)(threadIdx.x & 31) : (int)(threadIdx.x >> 3);
   unsigned base, stride;
@@ -833,13 +846,20 @@ __global__ __launch_bounds__(256, 4) void k_gen_pass(const float2* src, float2* 
     if (MODE == 1) msq = fold_sq((int)outer, (int)L) + fold_sq((int)p, (int)L);
   }
   const float sc = MODE >= 2 ? bal[0] : 1.0f;
-  float2* line = gsm + l * plan.lp;
+  float2* line = gsm + l * (LC == 600 ? 616 : plan.lp);
   // the first radix is 8 on every generic side (multiples of 8)
-  gen_first<8, SIGN, MODE>(plan, jj, line, src, base, stride, msq, kappa);
+  gen_first<8, SIGN, MODE, LC>(plan, jj, line, src, base, stride, msq, kappa);
+  if constexpr (LC == 600) {
+    gen_mid<3, SIGN, true, 600, 8>(plan, 1, jj, line, tw);
+    gen_mid<5, SIGN, false, 600, 24>(plan, 2, jj, line, tw);
+    gen_last<5, SIGN, MODE, false, 600, 120>(plan, 3, jj, line, dst, base, stride, tw, sc, wmap,
+                                             d2);
+    return;
+  }
   const int last = plan.nst - 1;
-#define GMS(R, T) gen_mid<R, SIGN, T>(plan, st, jj, line, tw)
+#define GMS(R, T) gen_mid<R, SIGN, T, 0, 0>(plan, st, jj, line, tw)
 #define GLS(R, T) \
-  gen_last<R, SIGN, MODE, T>(plan, last, jj, line, dst, base, stride, tw, sc, wmap, d2)
+  gen_last<R, SIGN, MODE, T, 0, 0>(plan, last, jj, line, dst, base, stride, tw, sc, wmap, d2)
   if (last == 1) {
     switch (plan.radix[1]) {
       case 8: GLS(8, true); break;
@@ -1111,9 +1131,11 @@ bool gen_plan_make(int n, GenPlan* p) {
 
 size_t gen_pass_lds(const GenPlan& p) { return sizeof(float2) * kGenLines * (size_t)p.lp; }
 
-#define GEN_KERNELS(X)                                                                     \
-  X(-1, 0, true) X(-1, 0, false) X(+1, 0, true) X(+1, 0, false) X(+1, 1, true)             \
-  X(+1, 1, false) X(+1, 2, true) X(+1, 2, false) X(+1, 3, true) X(+1, 3, false)
+#define GEN_KERNELS_L(X, LC)                                                               \
+  X(-1, 0, true, LC) X(-1, 0, false, LC) X(+1, 0, true, LC) X(+1, 0, false, LC)             \
+  X(+1, 1, true, LC) X(+1, 1, false, LC) X(+1, 2, true, LC) X(+1, 2, false, LC)             \
+  X(+1, 3, true, LC) X(+1, 3, false, LC)
+#define GEN_KERNELS(X) GEN_KERNELS_L(X, 0) GEN_KERNELS_L(X, 600)
 
 bool gen_prepare(const GenPlan& p) {
   // LDS of the transform blocks above 64 KiB is requested per kernel
@@ -1122,8 +1144,8 @@ bool gen_prepare(const GenPlan& p) {
   const int v = (int)lds;
   const hipFuncAttribute at = hipFuncAttributeMaxDynamicSharedMemorySize;
   bool ok = true;
-#define GA(SG, MD, RW) \
-  ok = ok && hipFuncSetAttribute((const void*)k_gen_pass<SG, MD, RW>, at, v) == hipSuccess;
+#define GA(SG, MD, RW, LC) \
+  ok = ok && hipFuncSetAttribute((const void*)k_gen_pass<SG, MD, RW, LC>, at, v) == hipSuccess;
   GEN_KERNELS(GA)
 #undef GA
   return ok;
@@ -1137,10 +1159,12 @@ void launch_gen_pass(const GenPlan& plan, int axis, int sign, int mode, const fl
   const size_t lds = gen_pass_lds(plan);
   const bool rows = axis == 0;
   const int sg = sign < 0 ? -1 : +1, md = sign < 0 ? 0 : mode;
-#define GL(SG, MD, RW)                                                                        \
-  if (sg == SG && md == MD && rows == RW)                                                     \
-    hipLaunchKernelGGL((k_gen_pass<SG, MD, RW>), g, b, lds, s, src, dst, plan, axis, tw, kappa, \
-                       bal, wmap, d2);
+  // the specialised side's plan is the one gen_plan_make builds for it
+  const int lc = (plan.L == 600 && plan.nst == 4 && plan.lp == 616) ? 600 : 0;
+#define GL(SG, MD, RW, LC)                                                                     \
+  if (sg == SG && md == MD && rows == RW && lc == LC)                                          \
+    hipLaunchKernelGGL((k_gen_pass<SG, MD, RW, LC>), g, b, lds, s, src, dst, plan, axis, tw,    \
+                       kappa, bal, wmap, d2);
   GEN_KERNELS(GL)
 #undef GL
 }
