@@ -3,16 +3,17 @@ of 8 with prime factors 2, 3 and 5): mixed-radix transforms through LDS,
 one radius per step, then the same dilation and final pass as the plan
 sides.
 
-  * 80 (4*4*5), 120 (4*2*3*5), 200 (4*2*5*5): the fused parity bar against
-    the FP64 oracle (the xHII box equals the exact dilation of the GPU's own
-    centre map merged with the quick fill, bit for bit; centre-map
-    differences only at near-threshold cells; mean x_HII);
-  * 600 (4*2*3*5*5, the N_smoothed of the reference's sample
-    simfast21.ini; dilation lanes of 15 cells): a plane wave through the
-    filtered-field hook equals W(|k| R) times the wave (closed form), and
-    the xHII box of a synthetic slice equals the exact dilation of the
-    GPU's own centre map merged with the quick fill, bit for bit, with the
-    neutral fraction its FP64 mean."""
+  * 80 (8*2*5), 120 (8*3*5), 200 (8*5*5), 240 (8*2*3*5: a second middle
+    stage): the fused parity bar against the FP64 oracle (the xHII box
+    equals the exact dilation of the GPU's own centre map merged with the
+    quick fill, bit for bit; centre-map differences only at near-threshold
+    cells; mean x_HII);
+  * 600 (8*3*5*5, the N_smoothed of the reference's sample simfast21.ini,
+    with kernels specialised for it; dilation lanes of 15 cells): a plane
+    wave through the filtered-field hook equals W(|k| R) times the wave
+    (closed form), and the xHII box of a synthetic slice equals the exact
+    dilation of the GPU's own centre map merged with the quick fill, bit for
+    bit, with the neutral fraction its FP64 mean."""
 import numpy as np
 import pytest
 
@@ -23,7 +24,7 @@ from test_gpu_fused import check_vs_oracle, quick_fill_np
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("n,zeta", [(80, 1.0), (120, 1.0), (120, 3.0), (200, 1.0)])
+@pytest.mark.parametrize("n,zeta", [(80, 1.0), (120, 1.0), (120, 3.0), (200, 1.0), (240, 1.0)])
 def test_generic_vs_oracle(torch_cuda, n, zeta):
     case = direct_case(n, zeta)
     ctx = sf.HiiContext(n, case["side"], case["radii"], sf.INPUT_DIRECT, sf.PATH_FUSED)
