@@ -74,13 +74,15 @@ typedef struct sf21_hii_config {
   double hubble;         /* h, SFR mode (Rrec's cell volume, user_functions.c:62) */
   double fesc;           /* escape fraction, SFR mode (get_HIIbubbles.c:273) */
   int radius_batch;      /* radii per fused filter pass, 0 = automatic (16, fewer if the
-                            K n^3 complex buffer does not fit) */
+                            K n^3 complex buffer does not fit); sides without a register
+                            plan run one radius per step */
   int device;            /* HIP device ordinal */
   int filter_tile;       /* columns per block of the fused filter pass: 4, 8, 16 or 0 (auto) */
   int filter_w;          /* top-hat weights W(|k| R): 0 = automatic (FP32 in the kernel),
                             1 = FP32 evaluation in the kernel (series below |k|R = 1.25,
                             3 (sin x - x cos x)/x^3 above), 2 = FP64 table per integer
-                            |k|^2 rounded to float */
+                            |k|^2 rounded to float (sides without a register plan always
+                            evaluate in the kernel) */
   int n_halo;            /* N_halo, halo mode: the catalog grid (>= n) */
 } sf21_hii_config;
 
