@@ -41,8 +41,8 @@ struct GenPlan {
 // element of src times W(|k| R) (kappa = dk R as hi + lo) before the
 // transform, result to dst; 2: centre test after the transform (Re = n^3
 // rec_R, Im = s n^3 ion_R, s = bal[0]): wmap = d2 where a centre, nothing
-// stored; 3: dst = (Re, Im / s) (the unscaled filtered pair).  LDS:
-// gen_pass_lds(plan) bytes.
+// stored (dst null); 3: dst = (Re, Im / s), the unscaled filtered pair (the
+// same kernel as mode 2).  LDS: gen_pass_lds(plan) bytes.
 bool gen_plan_make(int n, GenPlan* p);
 size_t gen_pass_lds(const GenPlan& p);
 // allows the transform blocks' LDS on the current device (false: refused)

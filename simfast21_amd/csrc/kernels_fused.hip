@@ -775,8 +775,10 @@ __device__ __forceinline__ void gen_mid(const GenPlan& P, int st, int jj, float2
   __syncthreads();
 }
 
-// Last stage (NS R = L, k = j): outputs at j + r L/R to HBM (mode 0), the
-// centre test (mode 2) or the unscaled pair (mode 3).
+// Last stage (NS R = L, k = j): outputs at j + r L/R to HBM (mode 0), or in
+// mode 2 the centre test (dst null) or the unscaled pair to dst (the
+// inspection hook: one kernel for both, so that the hook sees the values the
+// centre test saw, bit for bit).
 template <int R, int SIGN, int MODE, bool T1, int LC, int NSC>
 __device__ __forceinline__ void gen_last(const GenPlan& P, int st, int jj, const float2* line,
                                          float2* dst, unsigned base, unsigned stride,
@@ -798,9 +800,10 @@ __device__ __forceinline__ void gen_last(const GenPlan& P, int st, int jj, const
       const unsigned idx = base + (unsigned)(j + r * Lr) * stride;
       const float2 v = a[r];
       if constexpr (MODE == 2) {
-        if (is_centre(v.x * sc, v.y)) wmap[idx] = d2;
-      } else if constexpr (MODE == 3) {
-        dst[idx] = make_float2(v.x, v.y / sc);
+        if (dst)
+          dst[idx] = make_float2(v.x, v.y / sc);
+        else if (is_centre(v.x * sc, v.y))
+          wmap[idx] = d2;
       } else {
         dst[idx] = v;
       }
@@ -1133,8 +1136,7 @@ size_t gen_pass_lds(const GenPlan& p) { return sizeof(float2) * kGenLines * (siz
 
 #define GEN_KERNELS_L(X, LC)                                                               \
   X(-1, 0, true, LC) X(-1, 0, false, LC) X(+1, 0, true, LC) X(+1, 0, false, LC)             \
-  X(+1, 1, true, LC) X(+1, 1, false, LC) X(+1, 2, true, LC) X(+1, 2, false, LC)             \
-  X(+1, 3, true, LC) X(+1, 3, false, LC)
+  X(+1, 1, true, LC) X(+1, 1, false, LC) X(+1, 2, true, LC) X(+1, 2, false, LC)
 #define GEN_KERNELS(X) GEN_KERNELS_L(X, 0) GEN_KERNELS_L(X, 600)
 
 bool gen_prepare(const GenPlan& p) {
@@ -1158,7 +1160,8 @@ void launch_gen_pass(const GenPlan& plan, int axis, int sign, int mode, const fl
   const dim3 g(L * L / kGenLines), b(256);
   const size_t lds = gen_pass_lds(plan);
   const bool rows = axis == 0;
-  const int sg = sign < 0 ? -1 : +1, md = sign < 0 ? 0 : mode;
+  // mode 3 (the unscaled pair) runs the centre-test kernel with dst set
+  const int sg = sign < 0 ? -1 : +1, md = sign < 0 ? 0 : (mode == 3 ? 2 : mode);
   // the specialised side's plan is the one gen_plan_make builds for it
   const int lc = (plan.L == 600 && plan.nst == 4 && plan.lp == 616) ? 600 : 0;
 #define GL(SG, MD, RW, LC)                                                                     \
