@@ -735,6 +735,54 @@ __device__ __forceinline__ void gen_first(const GenPlan& P, int jj, float2* line
   __syncthreads();
 }
 
+// Filtered first stage (mode 1) of the i-axis pass at the 600 side: the W
+// of the block's 8 lines depends on |k|^2 = msq_l + fold(e)^2, so it is
+// evaluated once per (line, |fold(e)|) -- 8 x 301 values instead of 8 x 600
+// -- into a table in the line buffers, which the stage's stores overwrite
+// only after every thread has multiplied its inputs.  The same function of
+// the same |k|^2 as the direct evaluation: identical values.
+template <int SIGN>
+__device__ __forceinline__ void gen_first_wtab600(float2* gsm, float2* line, int l, int jj,
+                                                  const float2* src, unsigned base,
+                                                  unsigned stride, int mfix, int p0,
+                                                  float2 kappa) {
+  constexpr int L = 600, R = 8, Lr = L / R, H = L / 2 + 1, MAXIT = (Lr + 31) / 32;
+  float* wt = reinterpret_cast<float*>(gsm);
+  for (int t = threadIdx.x; t < kGenLines * H; t += blockDim.x) {
+    const int ll = t / H, f = t - ll * H;
+    wt[t] = w_tophat_k(kappa, mfix + fold_sq(p0 + ll, L) + f * f);
+  }
+  lds_pin();
+  __syncthreads();
+  float2 a[MAXIT][R];
+#pragma unroll
+  for (int it = 0; it < MAXIT; ++it) {
+    const int j = jj + 32 * it;
+    if (j < Lr) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int e = j + r * Lr;
+        const float2 v = src[base + (unsigned)e * stride];
+        const float w = wt[l * H + (e <= L / 2 ? e : L - e)];
+        a[it][r] = make_float2(v.x * w, v.y * w);
+      }
+    }
+  }
+  lds_pin();
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < MAXIT; ++it) {
+    const int j = jj + 32 * it;
+    if (j < Lr) {
+      Dft<R, SIGN>::run(a[it]);
+#pragma unroll
+      for (int r = 0; r < R; ++r) line[r * Lr + j] = a[it][r];
+    }
+  }
+  lds_pin();
+  __syncthreads();
+}
+
 // Middle stage: all of a thread's butterflies are read before the barrier,
 // then written back into the same line.  T1: the stage after the first
 // (inputs in the transposed slots).
@@ -851,7 +899,15 @@ __global__ __launch_bounds__(256, 4) void k_gen_pass(const float2* src, float2* 
   const float sc = MODE >= 2 ? bal[0] : 1.0f;
   float2* line = gsm + l * (LC == 600 ? 616 : plan.lp);
   // the first radix is 8 on every generic side (multiples of 8)
-  gen_first<8, SIGN, MODE, LC>(plan, jj, line, src, base, stride, msq, kappa);
+  if constexpr (LC == 600 && MODE == 1 && !ROWS) {
+    // the filtered pass runs along i only (axis 2: outer = j)
+    const unsigned ntile = L / kGenLines;
+    gen_first_wtab600<SIGN>(gsm, line, l, jj, src, base, stride,
+                            fold_sq((int)(blk / ntile), (int)L), (int)((blk % ntile) * kGenLines),
+                            kappa);
+  } else {
+    gen_first<8, SIGN, MODE, LC>(plan, jj, line, src, base, stride, msq, kappa);
+  }
   if constexpr (LC == 600) {
     gen_mid<3, SIGN, true, 600, 8>(plan, 1, jj, line, tw);
     gen_mid<5, SIGN, false, 600, 24>(plan, 2, jj, line, tw);
@@ -1163,7 +1219,9 @@ void launch_gen_pass(const GenPlan& plan, int axis, int sign, int mode, const fl
   // mode 3 (the unscaled pair) runs the centre-test kernel with dst set
   const int sg = sign < 0 ? -1 : +1, md = sign < 0 ? 0 : (mode == 3 ? 2 : mode);
   // the specialised side's plan is the one gen_plan_make builds for it
-  const int lc = (plan.L == 600 && plan.nst == 4 && plan.lp == 616) ? 600 : 0;
+  // (mode 1 runs along i only; its specialisation takes outer = j)
+  const int lc =
+      (plan.L == 600 && plan.nst == 4 && plan.lp == 616 && !(md == 1 && axis != 2)) ? 600 : 0;
 #define GL(SG, MD, RW, LC)                                                                     \
   if (sg == SG && md == MD && rows == RW && lc == LC)                                          \
     hipLaunchKernelGGL((k_gen_pass<SG, MD, RW, LC>), g, b, lds, s, src, dst, plan, axis, tw,    \
