@@ -115,6 +115,39 @@ def test_create_without_gpu_fails_cleanly():
     assert sf._lib().sf21_hii_create(ctypes.byref(cfg), ctypes.byref(h)) == sf.SF21_ERR_ARG
 
 
+def _fused_side_ok(n):
+    """The fused finder's side rule restated: a register-plan side, or a
+    multiple of 8 in 16..768 with prime factors 2, 3, 5 and some q <= 16 with
+    n/q a multiple of 8 and at most 64 (the dilation's lanes)."""
+    if n in (32, 64, 96, 128, 192, 256, 512):
+        return True
+    if n < 16 or n > 768 or n % 8:
+        return False
+    m = n
+    for f in (2, 3, 5):
+        while m % f == 0:
+            m //= f
+    return m == 1 and any(n % q == 0 and n // q <= 64 and (n // q) % 8 == 0 for q in range(1, 17))
+
+
+def test_size_rule_over_all_sides():
+    """Every side 8..1032 in steps of 8 (and a few odd ones): accepted by
+    the library exactly when the rule above holds."""
+    if _have_gpu():
+        pytest.skip("a HIP device is present")
+    radii = (ctypes.c_double * 2)(4.0, 2.0)
+    h = ctypes.c_void_p()
+    accepted = []
+    for n in list(range(8, 1040, 8)) + [60, 75, 100, 150]:
+        cfg = sf.HiiConfig(n, 100.0, 2, radii, sf.INPUT_DIRECT, sf.PATH_FUSED, 0.7, 1.0, 0, 0, 0, 0, 0)
+        rc = sf._lib().sf21_hii_create(ctypes.byref(cfg), ctypes.byref(h))
+        assert rc in (sf.SF21_ERR_HIP, sf.SF21_ERR_ARG)
+        assert (rc == sf.SF21_ERR_HIP) == _fused_side_ok(n), n
+        if rc == sf.SF21_ERR_HIP:
+            accepted.append(n)
+    assert 600 in accepted and 768 in accepted and 1024 not in accepted
+
+
 @pytest.mark.parametrize("n,path,ok", [
     (600, "FUSED", True), (80, "FUSED", True), (120, "FUSED", True), (200, "FUSED", True),
     (360, "FUSED", True), (768, "FUSED", True), (600, "PLAIN", False), (56, "FUSED", False),
