@@ -144,3 +144,64 @@ def test_generic_batch_graph_equals_single_runs(torch_cuda):
             assert np.array_equal(outs[k].cpu().numpy(), single[k][0]), (rep, k)
             assert neus[k] == single[k][1], (rep, k)
     ctx.close()
+
+
+SAMPLE_INI = """# the reference's sample simfast21.ini, bubble-finder keys, one redshift
+nthreads = 40
+zmax = 15
+zmin = 15
+dz = 1.0
+sim_length = 150
+hubble = 0.71
+omega_matter = 0.27
+omega_baryon = 0.046
+omega_lambda = 0.73
+N_halo = 1200
+N_smoothed = 600
+fesc = 0.25
+Ion_cutoff = 0.999
+bubble_Rmax = 60
+bubble_Nbins = 50
+use_SFR = T
+"""
+
+
+@pytest.mark.gpu
+def test_cli_sample_ini_600(tmp_path, torch_cuda):
+    """The drop-in CLI on the reference's sample configuration (N_smoothed =
+    600, sim_length 150, h 0.71, bubble_Rmax 60 with 50 bins, SFR mode) at
+    z = 15 with synthetic deltanl and SFRD boxes: it runs, writes the xHII box
+    and the text tables, and its box equals, bit for bit, the library's run
+    on the same inputs with the ini's derived parameters (box side, ladder);
+    the x_HI table holds the box's FP64 mean neutral fraction."""
+    import subprocess
+    from test_gpu_modes import sfr_case
+    torch = torch_cuda
+    n, z = 600, 15.0
+    (tmp_path / "delta").mkdir()
+    (tmp_path / "SFR").mkdir()
+    (tmp_path / "simfast21.ini").write_text(SAMPLE_INI)
+    c = sfr_case(n=n, z=z, seed=77, zeta=2.0, hubble=0.71, side_mpc=150.0, fesc=0.25)
+    c["delta"].astype(np.float32).tofile(tmp_path / "delta" / f"deltanl_z{z:.3f}_N600_L150.0.dat")
+    c["sfrd"].astype(np.float32).tofile(tmp_path / "SFR" / f"sfrd_z{z:.3f}_N600_L150.0.dat")
+    r = subprocess.run([str(sf.CLI_PATH), str(tmp_path)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.count("neutral fraction=") == 1
+    x_cli = np.fromfile(tmp_path / "Ionization" / f"xHII_z{z:.3f}_N600_L150.0.dat", np.float32)
+    assert x_cli.size == n ** 3
+    assert x_cli.min() >= 0.0 and x_cli.max() <= 1.0
+    xhi = np.array((tmp_path / "Output_text_files" / "x_HI_N600_L150.0.dat").read_text().split(), float)
+    assert xhi.size == 1
+    mean_neutral = 1.0 - x_cli.astype(np.float64).mean()
+    assert abs(xhi[0] - mean_neutral) < 2e-6
+    assert 0.0 < mean_neutral < 1.0
+    # the library with the ini's derived values
+    p = sf.read_params(tmp_path)
+    radii, _ = sf.ladder(p["bubble_rmax"], p["dx"], p["bubble_nbins"])
+    ctx = sf.HiiContext(n, p["box_len"], radii, sf.INPUT_SFR, sf.PATH_FUSED, hubble=p["hubble"],
+                        fesc=p["fesc"])
+    x_lib, neu = ctx.run_slice(torch.from_numpy(c["delta"].astype(np.float32)).cuda(),
+                               torch.from_numpy(c["sfrd"]).cuda(), z=z)
+    ctx.close()
+    print(f"sample ini at 600: {len(radii)} radii, mean x_HII {1 - neu:.6f}", flush=True)
+    assert np.array_equal(x_lib.cpu().numpy(), x_cli)
