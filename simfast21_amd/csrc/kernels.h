@@ -37,9 +37,9 @@ struct GenPlan {
 // Whole-line transforms (first and last stage in HBM, the others through
 // LDS), 8 lines per 256-thread block: axis 0
 // along p (contiguous rows), 1 along j, 2 along i (8 adjacent p columns).
-// mode 0: in place (src == dst allowed); 1: inverse-filter input -- each
-// element of src times W(|k| R) (kappa = dk R as hi + lo) before the
-// transform, result to dst; 2: centre test after the transform (Re = n^3
+// mode 0: in place (src == dst allowed); 1 (axis 2 only): inverse-filter
+// input -- each element of src times W(|k| R) (kappa = dk R as hi + lo)
+// before the transform, result to dst; 2: centre test after the transform (Re = n^3
 // rec_R, Im = s n^3 ion_R, s = bal[0]): wmap = d2 where a centre, nothing
 // stored (dst null); 3: dst = (Re, Im / s), the unscaled filtered pair (the
 // same kernel as mode 2).  LDS: gen_pass_lds(plan) bytes.

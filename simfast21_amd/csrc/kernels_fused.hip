@@ -735,18 +735,19 @@ __device__ __forceinline__ void gen_first(const GenPlan& P, int jj, float2* line
   __syncthreads();
 }
 
-// Filtered first stage (mode 1) of the i-axis pass at the 600 side: the W
-// of the block's 8 lines depends on |k|^2 = msq_l + fold(e)^2, so it is
-// evaluated once per (line, |fold(e)|) -- 8 x 301 values instead of 8 x 600
-// -- into a table in the line buffers, which the stage's stores overwrite
-// only after every thread has multiplied its inputs.  The same function of
-// the same |k|^2 as the direct evaluation: identical values.
-template <int SIGN>
-__device__ __forceinline__ void gen_first_wtab600(float2* gsm, float2* line, int l, int jj,
-                                                  const float2* src, unsigned base,
-                                                  unsigned stride, int mfix, int p0,
-                                                  float2 kappa) {
-  constexpr int L = 600, R = 8, Lr = L / R, H = L / 2 + 1, MAXIT = (Lr + 31) / 32;
+// Filtered first stage (mode 1) of the i-axis pass: the W of the block's 8
+// lines depends on |k|^2 = msq_l + fold(e)^2, so it is evaluated once per
+// (line, |fold(e)|) -- 8 (L/2 + 1) values instead of 8 L -- into a table in
+// the line buffers, which the stage's stores overwrite only after every
+// thread has multiplied its inputs.  The same function of the same |k|^2 as
+// the direct evaluation: identical values.  The first radix is 8, so a
+// thread holds at most (768/8 + 31)/32 = 3 butterflies.
+template <int SIGN, int LC>
+__device__ __forceinline__ void gen_first_wtab(const GenPlan& P, float2* gsm, float2* line,
+                                               int l, int jj, const float2* src, unsigned base,
+                                               unsigned stride, int mfix, int p0, float2 kappa) {
+  constexpr int R = 8, MAXIT = ((LC ? LC : kGenMaxL) / R + 31) / 32;
+  const int L = LC ? LC : P.L, Lr = L / R, H = L / 2 + 1;
   float* wt = reinterpret_cast<float*>(gsm);
   for (int t = threadIdx.x; t < kGenLines * H; t += blockDim.x) {
     const int ll = t / H, f = t - ll * H;
@@ -899,12 +900,12 @@ __global__ __launch_bounds__(256, 4) void k_gen_pass(const float2* src, float2* 
   const float sc = MODE >= 2 ? bal[0] : 1.0f;
   float2* line = gsm + l * (LC == 600 ? 616 : plan.lp);
   // the first radix is 8 on every generic side (multiples of 8)
-  if constexpr (LC == 600 && MODE == 1 && !ROWS) {
+  if constexpr (MODE == 1 && !ROWS) {
     // the filtered pass runs along i only (axis 2: outer = j)
     const unsigned ntile = L / kGenLines;
-    gen_first_wtab600<SIGN>(gsm, line, l, jj, src, base, stride,
-                            fold_sq((int)(blk / ntile), (int)L), (int)((blk % ntile) * kGenLines),
-                            kappa);
+    gen_first_wtab<SIGN, LC>(plan, gsm, line, l, jj, src, base, stride,
+                             fold_sq((int)(blk / ntile), (int)L),
+                             (int)((blk % ntile) * kGenLines), kappa);
   } else {
     gen_first<8, SIGN, MODE, LC>(plan, jj, line, src, base, stride, msq, kappa);
   }
@@ -1219,9 +1220,7 @@ void launch_gen_pass(const GenPlan& plan, int axis, int sign, int mode, const fl
   // mode 3 (the unscaled pair) runs the centre-test kernel with dst set
   const int sg = sign < 0 ? -1 : +1, md = sign < 0 ? 0 : (mode == 3 ? 2 : mode);
   // the specialised side's plan is the one gen_plan_make builds for it
-  // (mode 1 runs along i only; its specialisation takes outer = j)
-  const int lc =
-      (plan.L == 600 && plan.nst == 4 && plan.lp == 616 && !(md == 1 && axis != 2)) ? 600 : 0;
+  const int lc = (plan.L == 600 && plan.nst == 4 && plan.lp == 616) ? 600 : 0;
 #define GL(SG, MD, RW, LC)                                                                     \
   if (sg == SG && md == MD && rows == RW && lc == LC)                                          \
     hipLaunchKernelGGL((k_gen_pass<SG, MD, RW, LC>), g, b, lds, s, src, dst, plan, axis, tw,    \
