@@ -62,19 +62,23 @@ void launch_fwd_rows(int n, const float* a, const float* b, const float* c, Cell
                      const float* bal, float2* y, const float2* tw, hipStream_t s);
 // tile: columns per block (8, 16; 16 or 32 when wide); variant bit 0: 16
 // values per thread instead of the size's default (sizes 128, 256, 512);
-// bit 1: blocks walk 8 tiles, prefetching the next (tile 16, or 32 wide)
+// bit 1: blocks walk 8 tiles, prefetching the next (tile 16, or 32 wide);
+// bit 2: non-temporal loads and stores; bit 3 (axis 0, width n): the array
+// is in the 16-column tile layout -- each (i, 16 p columns) tile an n x 16
+// block, element (i, j, p) at i n^2 + (p/16) 16 n + 16 j + p%16
 void launch_cols(int n, int width, int axis, int sign, const float2* in, float2* out,
                  int nbatch, long bstride, const float2* tw, hipStream_t s, int tile = 16,
                  int variant = 0);
 // jpair: one block per mirror pair of i-lines (kj, n - kj), sharing W
-// (tile 8 or 16, weights evaluated in the kernel)
+// (tile 8 or 16, weights evaluated in the kernel); t16: u in the 16-column
+// tile layout of launch_cols (the p pass then reads it with t16 as well)
 void launch_filter_i(int n, const float2* spec, float2* u, int nrad, long ustride,
                      const float* wlut, long m_lut, const float2* kappa, const float2* tw,
-                     hipStream_t s, int tile, int wmode, bool jpair = false);
+                     hipStream_t s, int tile, int wmode, bool jpair = false, bool t16 = false);
 void launch_inv_rows_crit(int n, const float2* v, int nrad, long vstride, const int* d2max,
                           const float* bal, int* wmap, float2* dbg, hipStream_t s,
                           const float2* tw, bool wide = false,
-                          float2* keep = nullptr);
+                          float2* keep = nullptr, bool t16 = false);
 void launch_fill_i32(int* p, int value, long count, hipStream_t s);
 // cells per lane of the dilation line passes for box side n (0: n not usable)
 int dt_q(int n);

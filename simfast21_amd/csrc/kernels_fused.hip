@@ -187,9 +187,12 @@ __global__ __launch_bounds__(TILE*(L / E)) void k_cols(const float2* in, float2*
   if (axis == 0) {
     base = (long)outer * L * width + col;
     stride = width;
-  } else {
+  } else if (axis == 1) {
     base = (long)outer * width + col;
     stride = (long)L * width;
+  } else {  // 2: along j in the 16-column tile layout (width L)
+    base = (long)outer * L * width + (long)(col >> 4) * (L * 16) + (col & 15);
+    stride = 16;
   }
   base += (long)blockIdx.y * bstride;
   float* sre = lds + c * 2 * LdsLine<L>::kFloats;
@@ -234,7 +237,8 @@ __global__ __launch_bounds__(TILE*(L / E)) void k_cols_pipe(const float2* in, fl
   const long q1 = q0 + per_block < ntiles ? q0 + per_block : ntiles;
   float* sre = lds + c * 2 * LdsLine<L>::kFloats;
   float* sim = sre + LdsLine<L>::kFloats;
-  const long stride = axis == 0 ? (long)width : (long)L * width;
+  // axis 2: along j in the 16-column tile layout (width L)
+  const long stride = axis == 0 ? (long)width : (axis == 1 ? (long)L * width : 16L);
   // first element of column c of tile q, -1 past the last column
   auto origin = [&](long q) -> long {
     const long y = q / ((long)ntile * L);
@@ -242,7 +246,9 @@ __global__ __launch_bounds__(TILE*(L / E)) void k_cols_pipe(const float2* in, fl
     const int outer = (int)(rem / ntile);
     const int col = (int)(rem % ntile) * TILE + c;
     if (col >= width) return -1;
-    const long b = axis == 0 ? (long)outer * L * width + col : (long)outer * width + col;
+    const long b = axis == 0   ? (long)outer * L * width + col
+                   : axis == 1 ? (long)outer * width + col
+                               : (long)outer * L * width + (long)(col >> 4) * (L * 16) + (col & 15);
     return b + y * bstride;
   };
   float2 cur[E], nxt[E];
@@ -298,7 +304,7 @@ template <int L, int TILE, bool WTAB, bool JPAIR = false>
 __global__ __launch_bounds__(TILE*(L / Plan<L>::E)) void k_filter_i(
     const float2* __restrict__ spec, float2* __restrict__ u, int nrad, long ustride,
     const float* __restrict__ wlut, long mlut, const float2* __restrict__ kappa,
-    const float2* __restrict__ tw, int kj_step) {
+    const float2* __restrict__ tw, int kj_step, int t16) {
   constexpr int E = Plan<L>::E, TPL = L / E, NT = L / TILE;
   constexpr int NL = JPAIR ? 2 : 1;
   __shared__ float lds[TILE * 2 * LdsLine<L>::kFloats];
@@ -347,7 +353,11 @@ __global__ __launch_bounds__(TILE*(L / Plan<L>::E)) void k_filter_i(
 #pragma unroll
       for (int m = 0; m < E; ++m) v[m] = make_float2(z[l][m].x * wv[m], z[l][m].y * wv[m]);
       line_fft<L, +1>(v, sre, sim, tl, tw);
-      float2* __restrict__ dst = u + (long)r * ustride + (long)kjs[l] * L + kp;
+      // t16: the j pass's layout -- 16-column tiles of the (j, p) plane stored
+      // as contiguous L x 16 blocks, element (j, p) at (p/16) 16 L + 16 j + p%16
+      float2* __restrict__ dst =
+          u + (long)r * ustride +
+          (t16 ? (long)(kp >> 4) * (L * 16) + (long)kjs[l] * 16 + (kp & 15) : (long)kjs[l] * L + kp);
 #pragma unroll
       for (int m = 0; m < E; ++m) st_stream(dst + (tl + TPL * m) * (L * L), v[m]);
     }
@@ -372,7 +382,7 @@ __global__ __launch_bounds__(256) void k_inv_rows_crit(const float2* __restrict_
                                                        int* __restrict__ wmap,
                                                        float2* __restrict__ dbg,
                                                        float2* __restrict__ keep,
-                                                       const float2* __restrict__ tw) {
+                                                       const float2* __restrict__ tw, int t16) {
   constexpr int TPL = L / E, LPB = 256 / TPL;
   __shared__ float lds[LPB * 2 * LdsLine<L>::kFloats];
   const int lw = threadIdx.x / TPL, t = threadIdx.x % TPL;
@@ -386,10 +396,22 @@ __global__ __launch_bounds__(256) void k_inv_rows_crit(const float2* __restrict_
 #pragma unroll 1
   for (int r = 0; r < nrad; ++r) {
     const int tl = opaque_lane(t);
-    const float2* __restrict__ src = vin + (long)r * vstride + row * L;
     float2 v[E];
+    if (t16) {
+      // 16-column tile layout: element p of row (i, j) at i L^2 + (p/16) 16 L
+      // + 16 j + p%16
+      const float2* __restrict__ src =
+          vin + (long)r * vstride + (row / L) * ((long)L * L) + (row % L) * 16;
 #pragma unroll
-    for (int m = 0; m < E; ++m) v[m] = ld_stream(src + tl + TPL * m);
+      for (int m = 0; m < E; ++m) {
+        const int p = tl + TPL * m;
+        v[m] = ld_stream(src + (p >> 4) * (L * 16) + (p & 15));
+      }
+    } else {
+      const float2* __restrict__ src = vin + (long)r * vstride + row * L;
+#pragma unroll
+      for (int m = 0; m < E; ++m) v[m] = ld_stream(src + tl + TPL * m);
+    }
     line_fft_e<L, E, +1>(v, sre, sim, tl, tw);
     const int w = d2max[r];
 #pragma unroll
@@ -1032,6 +1054,7 @@ static void cols_t(int width, int axis, int sign, const float2* in, float2* out,
 void launch_cols(int n, int width, int axis, int sign, const float2* in, float2* out,
                  int nbatch, long bstride, const float2* tw, hipStream_t s, int tile, int variant) {
   const bool wide = variant & 1, pipe = (variant & 2) != 0, nt = (variant & 4) != 0;
+  if ((variant & 8) && axis == 0 && width == n) axis = 2;  // the 16-column tile layout
 #define X(L)                                                                              \
   if (n == L) {                                                                           \
     constexpr int E0 = Plan<L>::E;                                                        \
@@ -1061,7 +1084,8 @@ void launch_cols(int n, int width, int axis, int sign, const float2* in, float2*
 
 void launch_filter_i(int n, const float2* spec, float2* u, int nrad, long ustride,
                      const float* wlut, long m_lut, const float2* kappa, const float2* tw,
-                     hipStream_t s, int tile, int wmode, bool jpair) {
+                     hipStream_t s, int tile, int wmode, bool jpair, bool t16) {
+  const int lay = t16 ? 1 : 0;
 #define X(L, TL)                                                                          \
   if (n == L && tile == TL) {                                                             \
     constexpr int NT = TL * (L / Plan<L>::E);                                             \
@@ -1069,17 +1093,17 @@ void launch_filter_i(int n, const float2* spec, float2* u, int nrad, long ustrid
       /* mirror pairs (kj, L - kj), kj = 1 .. L/2 - 1, then the lines 0 and L/2 */        \
       const dim3 g((L / 2 - 1) * (L / TL)), g0(2 * (L / TL));                             \
       hipLaunchKernelGGL((k_filter_i<L, TL, false, true>), g, dim3(NT), 0, s, spec, u,    \
-                         nrad, ustride, wlut, m_lut, kappa, tw, 0);                       \
+                         nrad, ustride, wlut, m_lut, kappa, tw, 0, lay);                  \
       hipLaunchKernelGGL((k_filter_i<L, TL, false>), g0, dim3(NT), 0, s, spec, u, nrad,   \
-                         ustride, wlut, m_lut, kappa, tw, L / 2);                         \
+                         ustride, wlut, m_lut, kappa, tw, L / 2, lay);                    \
       return;                                                                             \
     }                                                                                     \
     if (wmode == 0)                                                                       \
       hipLaunchKernelGGL((k_filter_i<L, TL, true>), dim3(L * (L / TL)), dim3(NT), 0, s, spec, u, \
-                         nrad, ustride, wlut, m_lut, kappa, tw, 1);                       \
+                         nrad, ustride, wlut, m_lut, kappa, tw, 1, lay);                  \
     else                                                                                  \
       hipLaunchKernelGGL((k_filter_i<L, TL, false>), dim3(L * (L / TL)), dim3(NT), 0, s, spec, u, \
-                         nrad, ustride, wlut, m_lut, kappa, tw, 1);                       \
+                         nrad, ustride, wlut, m_lut, kappa, tw, 1, lay);                  \
     return;                                                                               \
   }
 #define Y(L) X(L, 4) X(L, 8) X(L, 16)
@@ -1090,19 +1114,20 @@ void launch_filter_i(int n, const float2* spec, float2* u, int nrad, long ustrid
 
 void launch_inv_rows_crit(int n, const float2* v, int nrad, long vstride, const int* d2max,
                           const float* bal, int* wmap, float2* dbg, hipStream_t s,
-                          const float2* tw, bool wide, float2* keep) {
+                          const float2* tw, bool wide, float2* keep, bool t16) {
+  const int lay = t16 ? 1 : 0;
 #define XW(L)                                                                             \
   if (n == L && wide) {                                                                   \
     constexpr int LPB = 256 / (L / 16);                                                   \
     hipLaunchKernelGGL((k_inv_rows_crit<L, 16>), dim3((L * L) / LPB), dim3(256), 0, s, v, nrad, \
-                       vstride, d2max, bal, wmap, dbg, keep, tw);                               \
+                       vstride, d2max, bal, wmap, dbg, keep, tw, lay);                          \
     return;                                                                               \
   }
 #define X(L)                                                                              \
   if (n == L) {                                                                           \
     constexpr int LPB = 256 / (L / Plan<L>::E);                                           \
     hipLaunchKernelGGL(k_inv_rows_crit<L>, dim3((L * L) / LPB), dim3(256), 0, s, v, nrad, vstride, \
-                       d2max, bal, wmap, dbg, keep, tw);                                        \
+                       d2max, bal, wmap, dbg, keep, tw, lay);                                   \
     return;                                                                               \
   }
   XW(128) XW(256) XW(512)

@@ -88,6 +88,10 @@ struct GraphEntry {
 // loads and stores carry the non-temporal hint.  The half-spectrum passes
 // (width n/2 + 1, tiles straddling cache lines) do not use it.
 constexpr int kColsStream = 4;
+// Column-pass variant bit: the per-radius buffer of the forward finder is in
+// the 16-column tile layout (the filter pass writes it, the j pass runs on
+// contiguous n x 16 blocks, the p pass reads 128 B pieces of its rows).
+constexpr int kColsT16 = 8;
 
 struct sf21_hii_ctx {
   sf21_hii_config cfg{};
@@ -354,13 +358,16 @@ void enqueue_fused(sf21_hii_ctx* c, const float* a, const float* b, const float*
     const int kk = re - rb;
     launch(c, "filter_i", [&] {
       launch_filter_i(n, c->spec, c->ubuf, kk, n3, c->wlut + (long)rb * c->mlut, c->mlut,
-                      c->d_kappa + rb, c->tw, s, c->tile_filter, c->filter_w, c->filter_pair);
+                      c->d_kappa + rb, c->tw, s, c->tile_filter, c->filter_w, c->filter_pair,
+                      true);
     });
-    launch(c, "cols_inv_j",
-           [&] { launch_cols(n, n, 0, +1, c->ubuf, c->ubuf, kk, n3, c->tw, s, c->cols_tile, c->cols_wide | kColsStream); });
+    launch(c, "cols_inv_j", [&] {
+      launch_cols(n, n, 0, +1, c->ubuf, c->ubuf, kk, n3, c->tw, s, c->cols_tile,
+                  c->cols_wide | kColsStream | kColsT16);
+    });
     launch(c, "inv_rows_crit", [&] {
       launch_inv_rows_crit(n, c->ubuf, kk, n3, c->d_d2 + rb, c->bal, c->wmap, dbg, s, c->tw, c->rows_wide,
-                           keep ? keep + (long)rb * n3 : nullptr);
+                           keep ? keep + (long)rb * n3 : nullptr, true);
     });
   }
   if (only_radius >= 0) return;
