@@ -106,7 +106,7 @@ zmin = 7
 dz = 1.0
 sim_length = 100
 hubble = 0.7
-N_halo = 128
+N_halo = {nh}
 N_smoothed = {n}
 fesc = 0.25
 Ion_cutoff = 0.999
@@ -127,7 +127,7 @@ def test_cli_end_to_end_vs_oracle_cli(tmp_path, torch_cuda, n):
         d = tmp_path / who
         (d / "delta").mkdir(parents=True)
         (d / "SFR").mkdir()
-        (d / "simfast21.ini").write_text(INI.format(n=n))
+        (d / "simfast21.ini").write_text(INI.format(n=n, nh=128))
         for k, z in enumerate((7.0, 8.0, 9.0)):
             c = sfr_case(n=n, z=z, seed=20 + k, zeta=1.0 + k)
             c["delta"].astype(np.float32).tofile(d / "delta" / f"deltanl_z{z:.3f}_N{n}_L100.0.dat")
@@ -194,8 +194,11 @@ def test_halo_emissivity_vs_oracle(torch_cuda, n, n_halo):
 
 
 @pytest.mark.gpu
-def test_cli_halo_mode_vs_oracle_cli(tmp_path, torch_cuda):
-    ini = INI.format(n=64).replace("use_SFR = T", "use_SFR = F")
+@pytest.mark.parametrize("n,nh", [(64, 128), (80, 160)])
+def test_cli_halo_mode_vs_oracle_cli(tmp_path, torch_cuda, n, nh):
+    """use_SFR = F through the CLIs (catalog CIC + smooth_box + finder) on the
+    same base_dir: x_HI tables within 1e-5.  80 has no register plan."""
+    ini = INI.format(n=n, nh=nh).replace("use_SFR = T", "use_SFR = F")
     dirs = {}
     for who in ("gpu", "oracle"):
         d = tmp_path / who
@@ -203,10 +206,10 @@ def test_cli_halo_mode_vs_oracle_cli(tmp_path, torch_cuda):
         (d / "Halos").mkdir()
         (d / "simfast21.ini").write_text(ini)
         for k, z in enumerate((7.0, 8.0, 9.0)):
-            c = sfr_case(n=64, z=z, seed=40 + k)
-            c["delta"].astype(np.float32).tofile(d / "delta" / f"deltanl_z{z:.3f}_N64_L100.0.dat")
-            cat = halo_catalog(128, 3000 * (k + 1), 50 + k)
-            with open(d / "Halos" / f"halonl_z{z:.3f}_N128_L100.0.dat.catalog", "wb") as f:
+            c = sfr_case(n=n, z=z, seed=40 + k)
+            c["delta"].astype(np.float32).tofile(d / "delta" / f"deltanl_z{z:.3f}_N{n}_L100.0.dat")
+            cat = halo_catalog(nh, 3000 * (k + 1), 50 + k)
+            with open(d / "Halos" / f"halonl_z{z:.3f}_N{nh}_L100.0.dat.catalog", "wb") as f:
                 np.array([len(cat)], np.int64).tofile(f)
                 cat.tofile(f)
         dirs[who] = d
@@ -216,8 +219,8 @@ def test_cli_halo_mode_vs_oracle_cli(tmp_path, torch_cuda):
     env = dict(os.environ, ORC_PRECISION="64", ORC_DILATION="1")
     r2 = subprocess.run([orc.CLI, str(dirs["oracle"])], capture_output=True, text=True, env=env)
     assert r2.returncode == 0, r2.stdout
-    g = np.array((dirs["gpu"] / "Output_text_files" / "x_HI_N64_L100.0.dat").read_text().split(), float)
-    o = np.array((dirs["oracle"] / "Output_text_files" / "x_HI_N64_L100.0.dat").read_text().split(), float)
+    g = np.array((dirs["gpu"] / "Output_text_files" / f"x_HI_N{n}_L100.0.dat").read_text().split(), float)
+    o = np.array((dirs["oracle"] / "Output_text_files" / f"x_HI_N{n}_L100.0.dat").read_text().split(), float)
     print("x_HI gpu", g, "oracle", o)
     assert np.allclose(g, o, atol=1e-5, rtol=0)
 
