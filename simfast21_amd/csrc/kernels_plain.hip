@@ -5,29 +5,9 @@
 // (get_HIIbubbles.c:110-183, :290-291, :324-325, :372-373, :382).
 #include "fft_core.h"
 #include "kernels.h"
+#include "cells.h"
 
 namespace sf21 {
-
-__device__ __forceinline__ void cell_fields_p(const CellConsts& k, const float* __restrict__ a,
-                                              const float* __restrict__ b,
-                                              const float* __restrict__ c, long i, float& rec,
-                                              float& ion, float& fres) {
-  if (k.mode == 0) {
-    const float d = a[i];
-    const float D = 1.0f + d;
-    const float r = fmaf(d, k.c_fres, k.c_fres);
-    fres = 2.0f * r / (2.0f * r + 1.0f + sqrtf(4.0f * r + 1.0f));
-    const float x = powf(D / k.r2, k.r3);
-    const float y = x / (1.0f + x);
-    const float y2 = y * y;
-    rec = k.k_rec * (y2 * y2);
-    ion = b[i] * k.k_ion;
-  } else {
-    rec = a[i];
-    ion = b[i];
-    fres = c ? c[i] : 0.0f;
-  }
-}
 
 // ------------------------------------------------------------------------
 // Real rows: an L-point real transform is an L/2-point complex transform of
@@ -144,7 +124,7 @@ __global__ void k_cell_fields(long n3, const float* __restrict__ a, const float*
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n3;
        i += (long)gridDim.x * blockDim.x) {
     float r, o, f;
-    cell_fields_p(k, a, b, c, i, r, o, f);
+    cell_fields(k, a, b, c, i, r, o, f);
     rec[i] = r;
     ion[i] = o;
     fresid[i] = f;
