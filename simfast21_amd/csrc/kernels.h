@@ -62,7 +62,7 @@ void launch_fwd_rows(int n, const float* a, const float* b, const float* c, Cell
                      const float* bal, float2* y, const float2* tw, hipStream_t s);
 // tile: columns per block (8, 16; 16 or 32 when wide); variant bit 0: 16
 // values per thread instead of the size's default (sizes 128, 256, 512);
-// bit 1: blocks walk 8 tiles, prefetching the next (tile 16, or 32 wide);
+// bit 1: blocks walk 8 tiles, prefetching the next (tile 16, or 16 and 32 wide);
 // bit 2: non-temporal loads and stores; bit 3 (axis 0, width n): the array
 // is in the 16-column tile layout -- each (i, 16 p columns) tile an n x 16
 // block, element (i, j, p) at i n^2 + (p/16) 16 n + 16 j + p%16

@@ -715,8 +715,10 @@ void launch_cols(int n, int width, int axis, int sign, const float2* in, float2*
       cols_t<L, 32, 16, true>(width, axis, sign, in, out, nbatch, bstride, tw, s, pipe);  \
     else if (tile == 32)                                                                  \
       cols_t<L, 32, 16, false>(width, axis, sign, in, out, nbatch, bstride, tw, s, pipe); \
+    else if (nt)                                                                          \
+      cols_t<L, 16, 16, true>(width, axis, sign, in, out, nbatch, bstride, tw, s, pipe);   \
     else                                                                                  \
-      cols_t<L, 16, 16, false>(width, axis, sign, in, out, nbatch, bstride, tw, s, false); \
+      cols_t<L, 16, 16, false>(width, axis, sign, in, out, nbatch, bstride, tw, s, pipe);  \
     return;                                                                               \
   }
   XW(128) XW(256) XW(512)

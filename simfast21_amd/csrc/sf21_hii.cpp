@@ -832,7 +832,7 @@ bool wide_ok(int n) { return n == 128 || n == 256 || n == 512; }
 // with tile 16 (8 values) or tile 32 (16 values).
 int cols_variant(int n, int tile, int cw) {
   const bool wide = (cw & 1) && wide_ok(n) && (tile == 16 || tile == 32);
-  const bool pipe = (cw & 2) && (wide ? tile == 32 : tile == 16);
+  const bool pipe = (cw & 2) && (tile == 16 || (wide && tile == 32));
   return (wide ? 1 : 0) | (pipe ? 2 : 0);
 }
 
@@ -1398,7 +1398,7 @@ int sf21_hii_autotune(sf21_hii_ctx* c, const float* a, const float* b, const flo
   }
   {
     const TuneChoice base = best;
-    const int cols_opts[6][2] = {{8, 0}, {16, 0}, {16, 2}, {16, 1}, {32, 1}, {32, 3}};
+    const int cols_opts[7][2] = {{8, 0}, {16, 0}, {16, 2}, {16, 1}, {16, 3}, {32, 1}, {32, 3}};
     for (auto& o : cols_opts) {
       if ((o[1] & 1) && !wide_ok(c->n)) continue;
       TuneChoice t = base;

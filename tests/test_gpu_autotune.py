@@ -27,7 +27,8 @@ def test_autotune_records_and_reloads(torch_cuda, tmp_path):
     print(f"\nautotune {n}^3, {len(radii)} radii: {ms:.3f} ms/slice with {t}", flush=True)
     assert ms > 0
     assert t["filter_tile"] in (8, 16) and t["filter_w"] in (1, 2) and t["filter_pair"] in (0, 1)
-    assert (t["cols_tile"], t["cols_wide"]) in ((8, 0), (16, 0), (16, 2), (16, 1), (32, 1), (32, 3))
+    assert (t["cols_tile"], t["cols_wide"]) in ((8, 0), (16, 0), (16, 2), (16, 1), (16, 3), (32, 1),
+                                                (32, 3))
     assert t["radius_batch"] in (8, 12, 16, len(radii))
     lines = cache.read_text().strip().splitlines()
     assert len(lines) == 1 and f"n={n} radii={len(radii)}" in lines[0]
@@ -56,7 +57,7 @@ def test_autotune_records_and_reloads(torch_cuda, tmp_path):
 
 
 @pytest.mark.parametrize("cols,rows_wide,fp", [((16, 1), 1, 0), ((32, 1), 0, 1), ((8, 0), 1, 0),
-                                               ((16, 2), 0, 1), ((32, 3), 1, 0)])
+                                               ((16, 2), 0, 1), ((16, 3), 1, 1), ((32, 3), 1, 0)])
 def test_wide_variants_are_bit_identical(torch_cuda, tmp_path, cols, rows_wide, fp):
     """16 values per thread run the same butterflies on the same values as 8,
     the prefetching column blocks (cw bit 1) the same transforms on the same
