@@ -67,8 +67,9 @@ typedef struct sf21_hii_config {
                             autotuner and the differentiable mode */
   double box_len;        /* box side L (unit of the radii) */
   int n_radii;           /* ladder length */
-  const double *radii;   /* descending ladder (get_HIIbubbles.c:298-461); radii >= 2*dx are
-                            the main loop, [dx, 2*dx) the small loop */
+  const double *radii;   /* descending ladder (get_HIIbubbles.c:298-461), every radius > 0
+                            and none larger than the one before (else SF21_ERR_ARG);
+                            radii >= 2*dx are the main loop, [dx, 2*dx) the small loop */
   int input_mode;        /* SF21_INPUT_* */
   int path;              /* SF21_PATH_* */
   double hubble;         /* h, SFR mode (Rrec's cell volume, user_functions.c:62) */
@@ -202,9 +203,10 @@ int sf21_nldensity_dev(sf21_hii_ctx *ctx, const sf21_params *p, double z, const 
  * at most half of the free device memory.  Mode 3: the exact forward keeps
  * them (its centre-test pass writes them, +8 B per voxel and radius) and the
  * next backward reads them instead of recomputing its first sweep -- when
- * its (g, emis, zeta) carry the same FP64 fingerprints as the forward's
- * (weighted sums of g and emis; an input changed in place between the two
- * calls is detected unless the change preserves both sums); otherwise it
+ * its (g, emis, zeta) carry the same fingerprints as the forward's (64-bit
+ * sums of a hash of each value's bits mixed with its index: an input
+ * changed in place between the two calls, values swapped between cells
+ * included, is detected except with probability ~2^-64); otherwise it
  * recomputes as in mode 1.  The gradient is identical in every mode;
  * SF21_ERR_ALLOC if mode 1 or 3 does not fit. */
 int sf21_hii_diff_store_fields(sf21_hii_ctx *ctx, int mode);
@@ -222,6 +224,12 @@ int sf21_hii_diff_backward_dev(sf21_hii_ctx *ctx, const float *g, const float *e
  * largest radius at which the cell was a bubble centre in the last slice, -1
  * if never (host output, n^3 ints). */
 int sf21_hii_weight_map(sf21_hii_ctx *ctx, int *host_out);
+/* Number of hipGraphs captured so far with ctx as their owner (batches on
+ * ctx, multi-box submissions whose first context is ctx).  A cached graph
+ * is replayed only while every context it uses keeps its buffers, stream
+ * and kernel choices; after sf21_hii_autotune, sf21_hii_set_stream or a
+ * context's destruction, the next call captures again. */
+int sf21_hii_graph_captures(const sf21_hii_ctx *ctx, long *captures);
 /* Filtered pair of one ladder radius for a slice: out (device, 2*n^3
  * floats, interleaved) = n^3 * (W_R rec, W_R ion) as the unnormalised c2r of
  * get_HIIbubbles.c:324-325 gives them. */
@@ -230,20 +238,21 @@ int sf21_hii_filtered_dev(sf21_hii_ctx *ctx, const float *a, const float *b, con
 
 /* Autotuner (fused path; no counterpart in the reference, whose FFTW plans
  * are tuned by FFTW_MEASURE at get_HIIbubbles.c:110-183).  Times the slice
- * (a, b, c, z; device pointers) by a coordinate search over: radius batch
- * 16/12/8 (capped by the ladder; skipped when it does not fit), top-hat
- * weights in the kernel or from the FP64 table, filter tile 16/8, filter
- * blocks over single i-lines or mirror pairs (kj, n - kj) sharing W, the
- * per-radius column passes (8 values per thread with 8 or 16 columns per
- * block, or 16 values per thread with 16 or 32 columns, the latter for
- * n = 128, 256, 512; with 16 or 32 columns also as blocks that walk 8 tiles
- * and prefetch the next) and the centre-test pass (8 or 16 values per
- * thread).
+ * (a, b, c, z; device pointers) by a coordinate search over launch shapes
+ * and stage orders: radius batch 16/12/8 (capped by the ladder; skipped
+ * when it does not fit), filter tile 16/8, filter blocks over single
+ * i-lines or mirror pairs (kj, n - kj) sharing W, the per-radius column
+ * passes (8 values per thread with 8 or 16 columns per block, or 16 values
+ * per thread with 16 or 32 columns, the latter for n = 128, 256, 512; with
+ * 16 or 32 columns also as blocks that walk 8 tiles and prefetch the next)
+ * and the centre-test pass (8 or 16 values per thread).  The top-hat weight
+ * mode (filter_w) is never searched: it stays what the configuration says.
  * Keeps the fastest in the context and appends it to the cache file
  * (cache_path, else $SF21_TUNE_CACHE, else not recorded) keyed by
- * (device arch, n, number of radii).  A context created with radius_batch,
- * filter_tile and filter_w all 0 takes the cached choice for its key from
- * $SF21_TUNE_CACHE.  *best_ms (nullable): the winner's time per slice. */
+ * (device arch, n, box_len, the radii).  A context created with
+ * radius_batch, filter_tile and filter_w all 0 takes the cached choice for
+ * its key from $SF21_TUNE_CACHE.  *best_ms (nullable): the winner's time per
+ * slice. */
 int sf21_hii_autotune(sf21_hii_ctx *ctx, const float *a, const float *b, const float *c,
                       double z, const char *cache_path, double *best_ms);
 /* Current choices: filter tile, radius batch, top-hat weights (1 in the

@@ -118,6 +118,7 @@ def _lib():
     L.sf21_hii_run_slice_dev.argtypes = [vp, vp, vp, vp, d, vp, dp]
     L.sf21_hii_run_batch_dev.argtypes = [vp, i, vp, vp, vp, dp, vp, dp]
     L.sf21_hii_weight_map.argtypes = [vp, vp]
+    L.sf21_hii_graph_captures.argtypes = [vp, ctypes.POINTER(ctypes.c_long)]
     L.sf21_hii_run_multi_dev.argtypes = [i, vp, vp, vp, vp, dp, vp, dp]
     L.sf21_hii_filtered_dev.argtypes = [vp, vp, vp, vp, d, i, vp]
     L.sf21_hii_profile.argtypes = [vp, i]
@@ -399,6 +400,13 @@ class HiiContext:
         out = np.empty(self.n ** 3, np.int32)
         _check(_lib().sf21_hii_weight_map(self._h, out.ctypes.data), "sf21_hii_weight_map")
         return out
+
+    @property
+    def graph_captures(self) -> int:
+        """hipGraphs captured so far with this context as their owner."""
+        v = ctypes.c_long()
+        _check(_lib().sf21_hii_graph_captures(self._h, ctypes.byref(v)), "sf21_hii_graph_captures")
+        return v.value
 
     def filtered(self, a, b, c=None, z: float = 0.0, radius_index: int = 0):
         import torch

@@ -147,9 +147,10 @@ void launch_ball_fold(int n, const float2* spec, float* table, hipStream_t s);
 void launch_rows_phi(int n, float2* u, int nrad, long ustride, const float* bal, const double* st,
                      float inv_tau, const float2* tw, float2* pairs, long pstride, hipStream_t s,
                      bool kept = false);
-// FP64 fingerprints of g and emis into out[0], out[1] (part: eb(n3) pairs)
-void launch_fingerprint(long n3, const float* g, const float* emis, double* part, double* out,
-                        hipStream_t s);
+// 64-bit position-mixed hashes of g and emis into out[0], out[1] (part:
+// eb(n3) pairs of doubles, reused as 64-bit partials)
+void launch_fingerprint(long n3, const float* g, const float* emis, double* part,
+                        unsigned long long* out, hipStream_t s);
 void launch_rows_seed(int n, float2* u, const float2* v, int nrad, long ustride, long vstride,
                       const float* bal, const double* st, float inv_tau, double* rho_acc,
                       const float2* tw, const float2* pairs, long pstride, hipStream_t s);

@@ -86,25 +86,63 @@ __global__ __launch_bounds__(256) void k_sum_exp(long n3, const float* __restric
   }
 }
 
-// Fingerprint of the inputs (g, E): FP64 sums of g_i w_i and E_i w_i with
-// w_i = 1 + (i mod 8191) 2^-13, over fixed per-block partials (deterministic
-// for a given launch shape).  Used to tell whether the fields kept by a
-// forward pass belong to the inputs of the following backward pass.
+// Fingerprint of the inputs (g, E): per element a 64-bit mix (the
+// splitmix64 finaliser) of the value's bits with its index, summed modulo
+// 2^64.  The sum is exact and order-free, so the result does not depend on
+// the launch shape; a changed value, or two values swapped between cells,
+// changes it except with probability ~2^-64.  Tells whether the fields kept
+// by a forward pass belong to the inputs of the following backward pass.
+__device__ __forceinline__ unsigned long long mix64(unsigned long long x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+__device__ __forceinline__ void block_reduce2_u64(unsigned long long a, unsigned long long b,
+                                                  unsigned long long* r1, unsigned long long* r2) {
+  r1[threadIdx.x] = a;
+  r2[threadIdx.x] = b;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      r1[threadIdx.x] += r1[threadIdx.x + s];
+      r2[threadIdx.x] += r2[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+}
+
 __global__ __launch_bounds__(256) void k_fingerprint(long n3, const float* __restrict__ g,
                                                      const float* __restrict__ emis,
-                                                     double* __restrict__ part) {
-  __shared__ double r1[256], r2[256];
-  double a = 0.0, b = 0.0;
+                                                     unsigned long long* __restrict__ part) {
+  __shared__ unsigned long long r1[256], r2[256];
+  unsigned long long a = 0, b = 0;
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n3;
        i += (long)gridDim.x * blockDim.x) {
-    const double w = 1.0 + (double)(i % 8191) * (1.0 / 8192.0);
-    a += (double)g[i] * w;
-    b += (double)emis[i] * w;
+    const unsigned long long pos = (unsigned long long)i << 32;
+    a += mix64(pos | __float_as_uint(g[i]));
+    b += mix64((pos | __float_as_uint(emis[i])) ^ 0xD6E8FEB86659FD93ull);
   }
-  block_reduce2(a, b, r1, r2);
+  block_reduce2_u64(a, b, r1, r2);
   if (threadIdx.x == 0) {
     part[2 * blockIdx.x] = r1[0];
     part[2 * blockIdx.x + 1] = r2[0];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_fingerprint_final(const unsigned long long* __restrict__ part,
+                                                           int np, unsigned long long* __restrict__ out) {
+  __shared__ unsigned long long r1[256], r2[256];
+  unsigned long long a = 0, b = 0;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) {
+    a += part[2 * i];
+    b += part[2 * i + 1];
+  }
+  block_reduce2_u64(a, b, r1, r2);
+  if (threadIdx.x == 0) {
+    out[0] = r1[0];
+    out[1] = r2[0];
   }
 }
 
@@ -760,10 +798,12 @@ int rows_blocks(int n) {
   }
   return 0;
 }
-void launch_fingerprint(long n3, const float* g, const float* emis, double* part, double* out,
-                        hipStream_t s) {
-  hipLaunchKernelGGL(k_fingerprint, dim3(eb(n3)), dim3(256), 0, s, n3, g, emis, part);
-  hipLaunchKernelGGL(k_final2, dim3(1), dim3(256), 0, s, part, (int)eb(n3), out, 0, 1);
+void launch_fingerprint(long n3, const float* g, const float* emis, double* part,
+                        unsigned long long* out, hipStream_t s) {
+  // part holds 2 eb(n3) doubles; the same bytes carry the 64-bit partials
+  auto* pu = reinterpret_cast<unsigned long long*>(part);
+  hipLaunchKernelGGL(k_fingerprint, dim3(eb(n3)), dim3(256), 0, s, n3, g, emis, pu);
+  hipLaunchKernelGGL(k_fingerprint_final, dim3(1), dim3(256), 0, s, pu, (int)eb(n3), out);
 }
 void launch_rows_phi(int n, float2* u, int nrad, long ustride, const float* bal, const double* st,
                      float inv_tau, const float2* tw, float2* pairs, long pstride, hipStream_t s,

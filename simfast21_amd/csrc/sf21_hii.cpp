@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -79,8 +80,14 @@ struct Pending {
 struct GraphEntry {
   std::vector<const void*> ptrs;
   std::vector<double> zs;
+  // multi-box graphs: (id, generation) of every context the graph uses
+  std::vector<unsigned long long> ids;
   hipGraphExec_t exec = nullptr;
 };
+
+// Context ids are never reused, so a graph that names a destroyed context
+// matches no later call, even when a new context lands at the same address.
+std::atomic<unsigned long long> g_next_ctx_id{1};
 
 }  // namespace
 
@@ -152,9 +159,10 @@ struct sf21_hii_ctx {
   float2* df_pairs = nullptr; // field-storage mode: n_R x n^3 filtered pairs
   int df_store = 2;           // 0 recompute, 1 store, 2 store if it fits in half the free HBM,
                               // 3 the exact forward keeps them for the backward
-  double* df_fp = nullptr;    // input fingerprints (device): [0..1] kept, [2..3] current
+  unsigned long long* df_fp = nullptr;  // input hashes (device): [0..1] kept, [2..3] current
   bool df_kept = false;       // df_pairs hold the fields of the inputs fingerprinted below
-  double df_kept_fp[2] = {0.0, 0.0};
+  unsigned long long df_kept_fp[2] = {0, 0};
+  unsigned long long* df_fp_host = nullptr;  // pinned staging of the four hashes
   double df_kept_zeta = 0.0;
   bool df_last_kept = false;  // the last backward read forward-kept fields
   bool df_ready = false;
@@ -174,6 +182,10 @@ struct sf21_hii_ctx {
   // graphs
   std::vector<GraphEntry> graphs;
   bool capturing = false;
+  // identity for graphs that span several contexts: id is unique per
+  // context; gen changes whenever buffers, stream or kernel choices change
+  unsigned long long id = 0, gen = 0;
+  long captures = 0;  // graphs captured with this context as owner
 };
 
 struct sf21_fft_plan {
@@ -478,6 +490,8 @@ void free_all(sf21_hii_ctx* c) {
   dev_free(c->df_pairs);
   if (c->df_host) (void)hipHostFree(c->df_host);
   c->df_host = nullptr;
+  if (c->df_fp_host) (void)hipHostFree(c->df_fp_host);
+  c->df_fp_host = nullptr;
   c->df_ready = false;
   if (c->flag_host) (void)hipHostFree(c->flag_host);
   if (c->neutral_host) (void)hipHostFree(c->neutral_host);
@@ -570,6 +584,11 @@ int ensure_diff(sf21_hii_ctx* c) {
     return rc;
   if (hipHostMalloc(reinterpret_cast<void**>(&c->df_host), sizeof(double) * 256) != hipSuccess) {
     c->df_host = nullptr;
+    return SF21_ERR_ALLOC;
+  }
+  if (hipHostMalloc(reinterpret_cast<void**>(&c->df_fp_host), sizeof(unsigned long long) * 4) !=
+      hipSuccess) {
+    c->df_fp_host = nullptr;
     return SF21_ERR_ALLOC;
   }
   // ball spectra of every radius: FFT of the periodic ball indicator, folded
@@ -739,8 +758,11 @@ int diff_check(sf21_hii_ctx* c, const float* g, const float* emis, int nbins) {
 }
 
 // ---- autotuner: choices of the fused path, persisted per (arch, n, radii) ----
+// Only choices that change launch shapes and stage order are searched; the
+// top-hat weight mode (in-kernel FP32 or the FP64 table) changes values and
+// stays an explicit configuration setting.
 struct TuneChoice {
-  int tile = 16, K = 16, w = 1, ct = 16, cw = 0, rw = 0, fp = 0;
+  int tile = 16, K = 16, ct = 16, cw = 0, rw = 0, fp = 0;
 };
 
 std::string tune_key(const sf21_hii_ctx* c) {
@@ -750,8 +772,17 @@ std::string tune_key(const sf21_hii_ctx* c) {
     arch = prop.gcnArchName;
     arch = arch.substr(0, arch.find(':'));
   }
+  // FNV-1a over the ladder's bytes: a choice is reused only for the same
+  // box side, length unit and radii
+  unsigned long long hsh = 1469598103934665603ull;
+  for (double R : c->radii) {
+    unsigned char b[sizeof(double)];
+    std::memcpy(b, &R, sizeof(double));
+    for (unsigned char x : b) hsh = (hsh ^ x) * 1099511628211ull;
+  }
   char buf[256];
-  std::snprintf(buf, sizeof(buf), "%s n=%d radii=%zu", arch.c_str(), c->n, c->radii.size());
+  std::snprintf(buf, sizeof(buf), "%s n=%d L=%.17g radii=%zu ladder=%016llx", arch.c_str(), c->n,
+                c->L, c->radii.size(), hsh);
   return buf;
 }
 
@@ -761,8 +792,8 @@ const char* tune_path(const char* given) {
   return (e && *e) ? e : nullptr;
 }
 
-// Cache lines: "<key> | tile=T K=K w=W ct=C cw=0..3 rw=0|1 fp=0|1 | <ms> ms"
-// (fp may be absent: 0); the last match wins.
+// Cache lines: "<key> | tile=T K=K ct=C cw=0..3 rw=0|1 fp=0|1 | <ms> ms";
+// the last match wins.
 bool tune_lookup(const char* path, const std::string& key, TuneChoice* out) {
   FILE* f = path ? std::fopen(path, "r") : nullptr;
   if (!f) return false;
@@ -774,8 +805,8 @@ bool tune_lookup(const char* path, const std::string& key, TuneChoice* out) {
     std::string k(line, (size_t)(bar - line));
     while (!k.empty() && k.back() == ' ') k.pop_back();
     TuneChoice t;
-    if (k == key && std::sscanf(bar + 1, " tile=%d K=%d w=%d ct=%d cw=%d rw=%d fp=%d", &t.tile,
-                                &t.K, &t.w, &t.ct, &t.cw, &t.rw, &t.fp) >= 6) {
+    if (k == key && std::sscanf(bar + 1, " tile=%d K=%d ct=%d cw=%d rw=%d fp=%d", &t.tile,
+                                &t.K, &t.ct, &t.cw, &t.rw, &t.fp) == 6) {
       *out = t;
       found = true;
     }
@@ -787,15 +818,19 @@ bool tune_lookup(const char* path, const std::string& key, TuneChoice* out) {
 void tune_store(const char* path, const std::string& key, const TuneChoice& t, double ms) {
   FILE* f = path ? std::fopen(path, "a") : nullptr;
   if (!f) return;
-  std::fprintf(f, "%s | tile=%d K=%d w=%d ct=%d cw=%d rw=%d fp=%d | %.3f ms\n", key.c_str(),
-               t.tile, t.K, t.w, t.ct, t.cw, t.rw, t.fp, ms);
+  std::fprintf(f, "%s | tile=%d K=%d ct=%d cw=%d rw=%d fp=%d | %.3f ms\n", key.c_str(),
+               t.tile, t.K, t.ct, t.cw, t.rw, t.fp, ms);
   std::fclose(f);
 }
 
+// Called whenever a context's buffers, stream or kernel choices change: its
+// own graphs go, and the new generation keeps multi-box graphs cached on
+// other contexts from replaying the old state.
 void drop_graphs(sf21_hii_ctx* c) {
   for (auto& g : c->graphs)
     if (g.exec) (void)hipGraphExecDestroy(g.exec);
   c->graphs.clear();
+  c->gen++;
 }
 
 // Change the radius batch: the K n^3 pair buffer (and the differentiable
@@ -838,11 +873,10 @@ int cols_variant(int n, int tile, int cw) {
 
 void apply_choice(sf21_hii_ctx* c, const TuneChoice& t) {
   c->tile_filter = t.tile;
-  c->filter_w = t.w;
   c->cols_tile = t.ct;
   c->cols_wide = cols_variant(c->n, t.ct, t.cw);
   c->rows_wide = t.rw && wide_ok(c->n);
-  c->filter_pair = t.fp && t.tile >= 8 && t.w;
+  c->filter_pair = t.fp && t.tile >= 8 && c->filter_w;
   c->df_kept = false;  // kept fields were computed with the previous choices
   drop_graphs(c);
 }
@@ -859,6 +893,7 @@ int create_deposit(const sf21_hii_config* cfg, sf21_hii_ctx** out) {
   }
   if (hipSetDevice(cfg->device) != hipSuccess) return SF21_ERR_HIP;
   auto* c = new sf21_hii_ctx();
+  c->id = g_next_ctx_id++;
   c->cfg = *cfg;
   c->cfg.radii = nullptr;
   c->cfg.n_radii = 0;
@@ -899,7 +934,12 @@ int sf21_hii_create(const sf21_hii_config* cfg, sf21_hii_ctx** out) {
   if (cfg->input_mode == SF21_INPUT_HALO && (cfg->n_halo < cfg->n || cfg->n_halo > 4096))
     return SF21_ERR_ARG;
   if (cfg->path != SF21_PATH_FUSED && cfg->path != SF21_PATH_PLAIN) return SF21_ERR_ARG;
-  *out = nullptr;
+  // the ladder is descending (get_HIIbubbles.c:298-461 divides R each step):
+  // the fused path walks it in ascending batches and stores each batch's
+  // ball bound over the previous one, which holds only in that order
+  for (int r = 0; r < cfg->n_radii; ++r)
+    if (!(cfg->radii[r] > 0) || (r > 0 && !(cfg->radii[r] <= cfg->radii[r - 1])))
+      return SF21_ERR_ARG;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= cfg->device || cfg->device < 0) {
     (void)hipGetLastError();
@@ -907,6 +947,7 @@ int sf21_hii_create(const sf21_hii_config* cfg, sf21_hii_ctx** out) {
   }
   if (hipSetDevice(cfg->device) != hipSuccess) return SF21_ERR_HIP;
   auto* c = new sf21_hii_ctx();
+  c->id = g_next_ctx_id++;
   c->cfg = *cfg;
   c->radii.assign(cfg->radii, cfg->radii + cfg->n_radii);
   c->cfg.radii = c->radii.data();
@@ -925,7 +966,6 @@ int sf21_hii_create(const sf21_hii_config* cfg, sf21_hii_ctx** out) {
   int rc = SF21_OK;
   long wmax = -1;
   for (double R : c->radii) {
-    if (!(R > 0)) { delete c; return SF21_ERR_ARG; }
     long d = sf21_ball_d2max(R, c->dx);
     c->d2.push_back((int)d);
     if (d > wmax) wmax = d;
@@ -948,11 +988,10 @@ int sf21_hii_create(const sf21_hii_config* cfg, sf21_hii_ctx** out) {
   c->filter_pair = c->n >= 256 && c->tile_filter >= 8 && c->filter_w;
   if (tuned) {
     c->tile_filter = tc.tile;
-    c->filter_w = tc.w;
     c->cols_tile = tc.ct;
     c->cols_wide = cols_variant(c->n, tc.ct, tc.cw);
     c->rows_wide = tc.rw && wide_ok(c->n);
-    c->filter_pair = tc.fp && tc.tile >= 8 && tc.w;
+    c->filter_pair = tc.fp && tc.tile >= 8 && c->filter_w;
   }
   if (c->K > (int)c->radii.size()) c->K = (int)c->radii.size();
   if (c->generic) c->K = 1;  // one radius per step, in-kernel FP32 weights
@@ -1056,9 +1095,7 @@ int sf21_hii_destroy(sf21_hii_ctx* c) {
 int sf21_hii_set_stream(sf21_hii_ctx* c, void* stream) {
   if (!c) return SF21_ERR_ARG;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  for (auto& g : c->graphs)
-    if (g.exec) (void)hipGraphExecDestroy(g.exec);
-  c->graphs.clear();
+  drop_graphs(c);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
   c->own_stream = false;
   c->stream = reinterpret_cast<hipStream_t>(stream);
@@ -1159,6 +1196,7 @@ int sf21_hii_run_batch_dev(sf21_hii_ctx* c, int ns, const float* const* a, const
       g.zs = zs;
       hipError_t e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
       (void)hipGraphDestroy(graph);
+      c->captures++;
       if (e != hipSuccess) return SF21_ERR_HIP;
       if (c->graphs.size() >= 8) {
         (void)hipGraphExecDestroy(c->graphs.front().exec);
@@ -1226,8 +1264,10 @@ int sf21_hii_run_multi_dev(int nbox, sf21_hii_ctx* const* ctx, const float* cons
     // (contexts, pointers, redshifts)
     std::vector<const void*> key;
     std::vector<double> zs(z, z + nbox);
+    std::vector<unsigned long long> ids;
     for (int i = 0; i < nbox; ++i) {
-      key.push_back(ctx[i]);
+      ids.push_back(ctx[i]->id);
+      ids.push_back(ctx[i]->gen);
       key.push_back(a[i]);
       key.push_back(b[i]);
       key.push_back(cc ? cc[i] : nullptr);
@@ -1235,7 +1275,7 @@ int sf21_hii_run_multi_dev(int nbox, sf21_hii_ctx* const* ctx, const float* cons
     }
     GraphEntry* ge = nullptr;
     for (auto& g : c0->graphs)
-      if (g.ptrs == key && g.zs == zs) ge = &g;
+      if (g.ptrs == key && g.zs == zs && g.ids == ids) ge = &g;
     if (!ge) {
       hipGraph_t graph = nullptr;
       std::vector<hipEvent_t> ev(uniq.size() + 1, nullptr);
@@ -1266,8 +1306,10 @@ int sf21_hii_run_multi_dev(int nbox, sf21_hii_ctx* const* ctx, const float* cons
         GraphEntry g;
         g.ptrs = key;
         g.zs = zs;
+        g.ids = ids;
         hipError_t e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
+        c0->captures++;
         if (e != hipSuccess) {
           rc = SF21_ERR_HIP;
         } else {
@@ -1308,6 +1350,12 @@ int sf21_hii_weight_map(sf21_hii_ctx* c, int* host_out) {
   if (hipStreamSynchronize(c->stream) != hipSuccess) return SF21_ERR_HIP;
   if (hipMemcpy(host_out, c->wmap, sizeof(int) * c->n3, hipMemcpyDeviceToHost) != hipSuccess)
     return SF21_ERR_HIP;
+  return SF21_OK;
+}
+
+int sf21_hii_graph_captures(const sf21_hii_ctx* c, long* captures) {
+  if (!c || !captures) return SF21_ERR_ARG;
+  *captures = c->captures;
   return SF21_OK;
 }
 
@@ -1380,11 +1428,6 @@ int sf21_hii_autotune(sf21_hii_ctx* c, const float* a, const float* b, const flo
     TuneChoice t = best;
     t.K = std::min(K, nr);
     if (t.K != best.K) consider(t);
-  }
-  {
-    TuneChoice t = best;
-    t.w = 0;
-    consider(t);
   }
   {
     TuneChoice t = best;
@@ -1805,14 +1848,14 @@ int sf21_hii_diff_forward_dev(sf21_hii_ctx* c, const float* g, const float* emis
                      c->stream) != hipSuccess ||
       hipMemcpyAsync(c->df_host + DS_COUNT, c->pk_out, sizeof(double) * nbins,
                      hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
-      (keep && hipMemcpyAsync(c->df_host + 200, c->df_fp, sizeof(double) * 2, hipMemcpyDeviceToHost,
-                              c->stream) != hipSuccess) ||
+      (keep && hipMemcpyAsync(c->df_fp_host, c->df_fp, sizeof(unsigned long long) * 2,
+                              hipMemcpyDeviceToHost, c->stream) != hipSuccess) ||
       hipStreamSynchronize(c->stream) != hipSuccess)
     return SF21_ERR_HIP;
   if (keep) {
     c->df_kept = true;
-    c->df_kept_fp[0] = c->df_host[200];
-    c->df_kept_fp[1] = c->df_host[201];
+    c->df_kept_fp[0] = c->df_fp_host[0];
+    c->df_kept_fp[1] = c->df_fp_host[1];
     c->df_kept_zeta = zeta;
   }
   if (mean_x) *mean_x = c->df_host[DS_SUM_X] / (double)c->n3;
@@ -1840,14 +1883,14 @@ int sf21_hii_diff_backward_dev(sf21_hii_ctx* c, const float* g, const float* emi
   bool kept = false;
   if (c->df_store == 3 && c->df_pairs) {
     launch_fingerprint(c->n3, g, emis, c->df_part, c->df_fp + 2, c->stream);
-    if (hipMemcpyAsync(c->df_host + 202, c->df_fp + 2, sizeof(double) * 2, hipMemcpyDeviceToHost,
-                       c->stream) != hipSuccess ||
+    if (hipMemcpyAsync(c->df_fp_host + 2, c->df_fp + 2, sizeof(unsigned long long) * 2,
+                       hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
         hipStreamSynchronize(c->stream) != hipSuccess)
       return SF21_ERR_HIP;
-    kept = c->df_kept && zeta == c->df_kept_zeta && c->df_host[202] == c->df_kept_fp[0] &&
-           c->df_host[203] == c->df_kept_fp[1];
-    c->df_kept_fp[0] = c->df_host[202];
-    c->df_kept_fp[1] = c->df_host[203];
+    kept = c->df_kept && zeta == c->df_kept_zeta && c->df_fp_host[2] == c->df_kept_fp[0] &&
+           c->df_fp_host[3] == c->df_kept_fp[1];
+    c->df_kept_fp[0] = c->df_fp_host[2];
+    c->df_kept_fp[1] = c->df_fp_host[3];
     c->df_kept_zeta = zeta;
   }
   c->df_kept = false;  // set again below once the pairs are this input's

@@ -115,6 +115,10 @@ int main(int argc, char *argv[]) {
     printf("%s", msg);
     exit(1);
   }
+  /* the reference is built with -D_OMPTHREAD_ (makefile:14) and reports its
+   * thread count here (get_HIIbubbles.c:74-79) */
+  printf("Using %d threads\n", p.nthreads);
+  fflush(0);
   make_dir(argv[1], "Ionization", "Creating Ionization directory\n");
   make_dir(argv[1], "Output_text_files", "Creating Output_text_files directory\n");
 

@@ -72,6 +72,7 @@ int main(int argc, char *argv[]) {
   lmpc = p.box_len / p.hubble;
   n3 = p.n * p.n * p.n;
   printf("\nCalculating SFRD between z=%f and z=%f with step %f\n", zmin, zmax, dz);
+  printf("Using %d threads\n", p.nthreads); /* get_SFR.c:59-62 with -D_OMPTHREAD_ */
   make_dir(argv[1], "SFR", "Creating SFR directory\n");
   make_dir(argv[1], "Output_text_files", "Creating Output_text_files directory\n");
   snprintf(fname, sizeof(fname), "%s/Output_text_files/sfrd_av_N%ld_L%.1f.dat", argv[1], p.n, lmpc);

@@ -79,6 +79,7 @@ int main(int argc, char *argv[]) {
   lmpc = p.box_len / p.hubble;
   n3 = p.n * p.n * p.n;
   n3h = p.n_halo * p.n_halo * p.n_halo;
+  printf("Using %d threads\n", p.nthreads); /* get_nldensity.c:51-54 with -D_OMPTHREAD_ */
   printf("\nzmin=%E zmax=%E  dz=%E\n", zmin, zmax, dz);
   printf("Reading velocity boxes...\n");
   fflush(0);

@@ -69,6 +69,7 @@ int main(int argc, char *argv[]) {
   }
   if (p.use_lya_xrays == 0)
     printf("Lya and xray use set to false - assuming TS>>TCMB in t21 calculation.\n");
+  printf("Using %d threads\n", p.nthreads); /* t21.c:60-65 with -D_OMPTHREAD_ */
   lmpc = p.box_len / p.hubble;
   n3 = p.n * p.n * p.n;
 

@@ -46,6 +46,29 @@ def test_library_exports_header_symbols():
         assert hasattr(lib, name), name
 
 
+@pytest.mark.parametrize("radii", [[2.0, 3.0, 5.0], [5.0, 2.0, 3.0], [5.0, 0.0], [5.0, float("nan")]])
+def test_create_rejects_unordered_ladder(radii):
+    """The ladder must be descending and positive (get_HIIbubbles.c:298-461
+    divides R each step); sf21_hii_create refuses anything else with
+    SF21_ERR_ARG before it touches a device."""
+    for path in (sf.PATH_FUSED, sf.PATH_PLAIN):
+        with pytest.raises(sf.Sf21Error) as e:
+            sf.HiiContext(64, 100.0, radii, sf.INPUT_DIRECT, path)
+        assert e.value.code == sf.SF21_ERR_ARG
+
+
+def test_create_accepts_equal_radii_past_argument_checks():
+    """Repeated radii are non-increasing and pass the argument checks: with
+    no device the call fails only at the device lookup."""
+    import torch
+    try:
+        ctx = sf.HiiContext(64, 100.0, [5.0, 5.0, 3.0], sf.INPUT_DIRECT, sf.PATH_FUSED)
+    except sf.Sf21Error as e:
+        assert not torch.cuda.is_available() and e.code == sf.SF21_ERR_HIP
+    else:
+        ctx.close()
+
+
 def test_params_derived_values(tmp_path):
     p = sf.read_params(write_ini(tmp_path))
     h = 0.7
@@ -181,3 +204,17 @@ def test_cli_usage_and_errors(tmp_path, cli):
     assert r.returncode == 1 and "Error reading deltanl file" in r.stdout
     assert (tmp_path / "Ionization").is_dir() and (tmp_path / "Output_text_files").is_dir()
     assert "Bubble radius ratio (bfactor)" in r.stdout and "z = 6.000000" in r.stdout
+
+
+@pytest.mark.parametrize("name", ["get_HIIbubbles", "get_SFR", "get_nldensity", "t21"])
+def test_cli_reports_thread_count(tmp_path, name):
+    """The reference programs are built with -D_OMPTHREAD_ (makefile:14) and
+    print "Using %d threads" with the ini's nthreads right after reading the
+    parameters (get_HIIbubbles.c:74-79, get_SFR.c:59-62,
+    get_nldensity.c:51-54, t21.c:60-65); the drop-in CLIs print the same line
+    before they touch any input (here there is none, so they then fail)."""
+    write_ini(tmp_path)
+    exe = os.path.join(os.path.dirname(str(sf.CLI_PATH)), name)
+    r = subprocess.run([exe, str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert "Using 4 threads\n" in r.stdout, r.stdout
+    assert r.returncode == 1
