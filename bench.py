@@ -6,11 +6,13 @@ float32, side 400 Mpc, 10 redshift slices, radii 100 Mpc * 0.9^k down to the
 cell size (47 radii), 2.5e5 point sources per slice.  Slice s uses the same
 Gaussian field g rescaled by its linear growth factor, g_s = (D_s/0.25) g with
 D_s = linspace(0.10, 0.25, 10), 1+delta_s = exp(g_s)/<exp(g_s)>, sources drawn
-with P ~ (1+delta_s)^2 and lognormal strengths, rec_s = (1+delta_s)^2.
+with P ~ (1+delta_s)^2 and lognormal strengths, rec_s = (1+delta_s)^2; each
+slice's emissivity is scaled (bisection, untimed) to a mean x_HII of ~0.5, as
+the configs' recipe asks.
 
 A step is one pass of all radii over all slices (inputs resident in HBM).
 `value` = slices * N^3 * n_radii / step time.  The plain per-radius GPU path
-(the reference's dataflow) is timed on one slice and scaled to ten slices.
+(the reference's dataflow) is timed on all slices, one at a time.
 The differentiable mode (one redshift at 512^3, same radii) reports the exact
 forward's time and the relaxed backward's, and their ratio, per tau.
 """
@@ -83,6 +85,31 @@ def make_inputs(torch, n, side, n_slices, n_src, seed=3001, slope=-2.5, sigma=1.
     return recs, ions
 
 
+def scale_slices(torch, ctx, recs, ions, target=0.5, tol=0.02):
+    """Efficiency per slice: each emissivity box is multiplied in place by
+    the zeta (bisection in log zeta on the fused path, untimed) that gives
+    the slice a mean x_HII of target within tol -- the configs' recipe
+    scales the emissivity to x_HII ~ 0.5.  Returns the zetas."""
+    zetas = []
+    out = torch.empty_like(recs[0])
+    for rec, ion in zip(recs, ions):
+        lo, hi = -6.0, 14.0
+        base = ion.clone()
+        zeta = 1.0
+        for _ in range(30):
+            mid = 0.5 * (lo + hi)
+            zeta = float(np.exp(mid))
+            torch.mul(base, zeta, out=ion)
+            _, neu = ctx.run_slice(rec, ion, xhii=out)
+            if abs(1 - neu - target) <= tol:
+                break
+            lo, hi = (mid, hi) if 1 - neu < target else (lo, mid)
+        zetas.append(zeta)
+        del base
+    torch.cuda.synchronize()
+    return zetas
+
+
 def measured_traffic(kernel, n, n_radii, radius_batch):
     """HBM bytes per launch of a kernel as measured by the rocprofv3 counter
     runs recorded under profiles/ (null when the workload differs)."""
@@ -97,25 +124,24 @@ def measured_traffic(kernel, n, n_radii, radius_batch):
     return rec.get("bytes_per_launch", {}).get(kernel)
 
 
-def bench_uneven(torch, sf, synth, n_boxes=64, steps=3, seed=4000, tune=True):
-    """Parameter-sweep batch: n_boxes boxes with N drawn uniformly from
-    {64, 96, 128, 192, 256} at a fixed cell size dx = 300/256 Mpc (sides
-    75-300 Mpc), P(k) slope ~ U[-2.8, -2.2], sigma 1.2, 4000 (N/128)^3
-    sources, efficiency zeta ~ 1.5 lognormal(0, 0.5); radius ladder per size
-    from R_max = side/4 down by 0.9 to the cell size.  All boxes in one
-    hipGraph launch (sf21_hii_run_multi_dev: boxes of one size in order,
-    sizes as parallel branches); launch choices autotuned per size on its
-    first box (not timed); per-size throughput from each box run alone."""
+UNEVEN_DX = 300.0 / 256
+
+
+def uneven_inputs(torch, synth, n_boxes=64, seed=4000):
+    """Inputs of the parameter-sweep batch: n_boxes boxes with N drawn
+    uniformly from {64, 96, 128, 192, 256} at a fixed cell size dx = 300/256
+    Mpc (sides 75-300 Mpc), P(k) slope ~ U[-2.8, -2.2], sigma 1.2,
+    4000 (N/128)^3 sources, efficiency zeta ~ 1.5 lognormal(0, 0.5); radius
+    ladder per size from R_max = side/4 down by 0.9 to the cell size.
+    Returns (sizes, ladders by size, rec boxes, emissivity boxes) with the
+    boxes on the device."""
     rng = np.random.default_rng(seed)
     gen = torch.Generator(device="cuda")
     gen.manual_seed(seed)
-    dx = 300.0 / 256
+    dx = UNEVEN_DX
     sizes = [int(v) for v in rng.choice([64, 96, 128, 192, 256], n_boxes)]
-    ctxs, ladders, a, b = {}, {}, [], []
-    for n in sorted(set(sizes)):
-        side = n * dx
-        ladders[n] = synth.geometric_ladder(side / 4, 0.9, dx)
-        ctxs[n] = sf.HiiContext(n, side, ladders[n], sf.INPUT_DIRECT, sf.PATH_FUSED)
+    ladders = {n: synth.geometric_ladder(n * dx / 4, 0.9, dx) for n in sorted(set(sizes))}
+    a, b = [], []
     for n in sizes:
         slope = float(rng.uniform(-2.8, -2.2))
         zeta = 1.5 * float(np.exp(0.5 * rng.standard_normal()))
@@ -125,6 +151,18 @@ def bench_uneven(torch, sf, synth, n_boxes=64, steps=3, seed=4000, tune=True):
         b.append((sources_dev(torch, gen, e, int(round(4000 * (n / 128) ** 3))) * zeta).contiguous())
         a.append((e ** 2).float().reshape(-1).contiguous())
         del g, e
+    return sizes, ladders, a, b
+
+
+def bench_uneven(torch, sf, synth, n_boxes=64, steps=3, seed=4000, tune=True):
+    """Parameter-sweep batch (uneven_inputs): all boxes in one hipGraph
+    launch (sf21_hii_run_multi_dev: boxes of one size in order, sizes as
+    parallel branches); launch choices autotuned per size on its first box
+    (not timed); per-size throughput from each box run alone."""
+    dx = UNEVEN_DX
+    sizes, ladders, a, b = uneven_inputs(torch, synth, n_boxes, seed)
+    ctxs = {n: sf.HiiContext(n, n * dx, ladders[n], sf.INPUT_DIRECT, sf.PATH_FUSED)
+            for n in sorted(set(sizes))}
     boxes = [ctxs[n] for n in sizes]
     if tune:
         for n, c in ctxs.items():
@@ -341,17 +379,7 @@ def bench_generic(torch, sf, synth, n=600, steps=2, seed=6006):
             "algorithmic_bytes_per_voxel_radius": 40, "kernels": kern}
 
 
-def cpu_baseline(n, side, radii_full, seconds_hint=20.0):
-    """Oracle (CPU restatement, reference dataflow, FP32 typing) on a bounded
-    sample of the workload: one 512^3 slice with the first radii of the ladder."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_bind as orc
-    from simfast21_amd import synth
-    delta, emis, rec = synth.direct_boxes(n, side, 250000, 3001, growth=0.175 / 0.25)
-    nr = 12
-    t0 = time.perf_counter()
-    orc.hii_slice(n, side, radii_full[:nr], rec, emis, precision=32, dilation=0)
-    dt = time.perf_counter() - t0
+def _host_cores():
     cores = os.cpu_count() or 1
     try:
         cores = len(os.sched_getaffinity(0))
@@ -360,15 +388,75 @@ def cpu_baseline(n, side, radii_full, seconds_hint=20.0):
     omp = os.environ.get("OMP_NUM_THREADS")
     if omp:
         cores = min(cores, int(omp))
+    return cores
+
+
+def cpu_baseline(n, side, radii_full, rec, ion):
+    """Oracle (CPU restatement, reference dataflow, FP32 typing, own OpenMP
+    FFT) on a bounded sample of the workload: one slice of the batch (host
+    copy of the timed inputs) with every fourth radius of the ladder, so the
+    sample spans the main loop (ball FFT convolutions) and the small loop
+    (stencil) as the full ladder does."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_bind as orc
+    sample = radii_full[::4]
+    t0 = time.perf_counter()
+    orc.hii_slice(n, side, sample, rec, ion, precision=32, dilation=0)
+    dt = time.perf_counter() - t0
+    n_main = sum(1 for R in sample if R >= 2 * side / n)
     return {
-        "value": n ** 3 * nr / dt,
+        "value": n ** 3 * len(sample) / dt,
         "unit": "voxel·radius updates/s",
-        "cores": cores,
+        "cores": _host_cores(),
         "kind": "port",
-        "sample": (f"one {n}^3 slice, first {nr} radii of the ladder (reference dataflow: 2 r2c + "
-                   f"per radius filter, 2 c2r, criterion, ball + 2 r2c + c2r), FP32 typing, "
-                   f"own OpenMP CPU FFT (not FFTW); {dt:.1f} s wall incl. forward transforms"),
+        "sample": (f"one {n}^3 slice of the batch (x_HII ~0.5), every 4th radius of the ladder "
+                   f"({len(sample)} of {len(radii_full)}: {n_main} main-loop, {len(sample) - n_main} "
+                   f"small-loop); reference dataflow (2 r2c + per radius filter, 2 c2r, criterion, "
+                   f"ball + 2 r2c + c2r or stencil), FP32 typing, own OpenMP CPU FFT (not FFTW); "
+                   f"{dt:.1f} s wall incl. the forward transforms"),
     }
+
+
+def cpu_configs(torch, sf, synth, repeats=3):
+    """CPU baselines of the two smaller configs, whole slices with all radii
+    (median of `repeats`): c1, 128^3 at side 200 Mpc, 4000 sources, radii
+    60 Mpc x 0.9^k, x_HII ~0.5; c2, 256^3 at side 300 Mpc, 30000 sources,
+    radii 80 Mpc x 0.9^k, at the recipe's floor (zeta 1e-3, x_HII ~0.31: 0.2
+    is below it) and x_HII ~0.5 and ~0.8 (zeta by bisection on the GPU)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_bind as orc
+    out = {"cores": _host_cores(), "kind": "port",
+           "method": ("oracle reference dataflow, FP32 typing, own OpenMP CPU FFT (not FFTW), "
+                      f"median of {repeats} whole slices with all radii")}
+    for name, n, side, n_src, rmax, seed, targets in (
+            ("c1", 128, 200.0, 4000, 60.0, 1001, [0.5]),
+            ("c2", 256, 300.0, 30000, 80.0, 2560, [None, 0.5, 0.8])):
+        delta, emis, rec = synth.direct_boxes(n, side, n_src, seed)
+        radii = synth.geometric_ladder(rmax, 0.9, side / n)
+        ctx = sf.HiiContext(n, side, radii, sf.INPUT_DIRECT, sf.PATH_FUSED)
+        a = torch.from_numpy(rec.ravel()).cuda()
+        for tgt in targets:
+            e = torch.from_numpy(emis.ravel()).cuda()
+            if tgt is None:
+                zeta = 1e-3
+                e.mul_(zeta)
+            else:
+                zeta = scale_slices(torch, ctx, [a], [e], target=tgt)[0]
+            ion = e.cpu().numpy()
+            _, neu = ctx.run_slice(a, e)
+            times = []
+            for _ in range(repeats):
+                t0 = time.perf_counter()
+                orc.hii_slice(n, side, radii, rec, ion, precision=32, dilation=0)
+                times.append(time.perf_counter() - t0)
+            med = float(np.median(times))
+            key = f"{name}_x{1 - neu:.2f}"
+            out[key] = {"n": n, "radii": len(radii), "zeta": round(zeta, 6),
+                        "mean_xhii_gpu": round(1 - neu, 4), "seconds": [round(t, 2) for t in times],
+                        "voxel_radius_per_s": round(n ** 3 * len(radii) / med, 1)}
+            print(f"[bench] cpu {key}: median {med:.2f} s", file=sys.stderr, flush=True)
+        ctx.close()
+    return out
 
 
 def main():
@@ -385,6 +473,8 @@ def main():
     ap.add_argument("--no-uneven", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--no-generic", action="store_true")
+    ap.add_argument("--no-cpu-configs", action="store_true",
+                    help="skip the CPU timings of the 128^3 and 256^3 configs")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one 1 GiB device copy (torch copy_) as the HBM-counter reference")
     args = ap.parse_args()
@@ -420,6 +510,9 @@ def main():
     ctx = sf.HiiContext(n, side, radii, sf.INPUT_DIRECT, sf.PATH_FUSED,
                         radius_batch=args.radius_batch)
     torch.cuda.synchronize()
+    # the recipe's efficiency: every slice scaled to mean x_HII ~0.5 (untimed)
+    zetas = scale_slices(torch, ctx, recs, ions)
+    print(f"[bench] slice efficiencies {[round(z, 3) for z in zetas]}", file=sys.stderr, flush=True)
     tuned = None
     if args.radius_batch <= 0 and not args.no_tune:
         # launch choices picked by the autotuner on one slice (not timed)
@@ -470,8 +563,12 @@ def main():
     roof = None
     if dom in algo:
         ach = algo[dom] / (avg_ms * 1e-3) / 1e9
+        traffic = measured_traffic(dom, n, len(radii), K)
         roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom, n, len(radii), K),
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "traffic_source": ("recorded: rocprofv3 FETCH_SIZE/WRITE_SIZE runs of this workload, "
+                                   "profiles/hbm_traffic.json (not measured by this run)")
+                if traffic is not None else None,
                 "kernel": dom, "avg_launch_ms": round(avg_ms, 4),
                 "algorithmic_bytes": algo[dom]}
     # whole-step algorithmic roofline (per voxel-radius 80h+4 B, per voxel-slice 80h+25 B)
@@ -485,18 +582,24 @@ def main():
     torch.cuda.empty_cache()
     plain = None
     if not args.no_plain:
+        # the reference's dataflow on the GPU, one slice at a time, all slices
         pctx = sf.HiiContext(n, side, radii, sf.INPUT_DIRECT, sf.PATH_PLAIN)
         pout = torch.empty_like(recs[0])
         pctx.run_slice(recs[S // 2], ions[S // 2], xhii=pout)
         torch.cuda.synchronize()
         tq0 = time.perf_counter()
-        _, pneu = pctx.run_slice(recs[S // 2], ions[S // 2], xhii=pout)
+        pneu = []
+        for k in range(S):
+            _, pn = pctx.run_slice(recs[k], ions[k], xhii=pout)
+            pneu.append(pn)
         torch.cuda.synchronize()
         pms = (time.perf_counter() - tq0) * 1e3
-        plain = {"ms_per_slice": round(pms, 2), "ms_per_step_10_slices_scaled": round(pms * S, 1),
-                 "speedup_fused_vs_plain": round(pms * S / ms, 2),
-                 "mean_xhii_slice": round(1 - pneu, 6), "fused_mean_xhii_slice": round(1 - neu[S // 2], 6)}
-        print(f"[bench] plain {pms:.1f} ms/slice", file=sys.stderr, flush=True)
+        plain = {"ms_per_step": round(pms, 1), "ms_per_slice": round(pms / S, 2),
+                 "speedup_fused_vs_plain": round(pms / ms, 2),
+                 "mean_xhii": [round(1 - v, 5) for v in pneu],
+                 "max_mean_diff_vs_fused": float(f"{max(abs(a - b) for a, b in zip(pneu, neu)):.3g}")}
+        print(f"[bench] plain {pms:.1f} ms/step ({pms / S:.1f} ms/slice)", file=sys.stderr, flush=True)
+        pctx.close()
         del pctx
 
     uneven = None
@@ -529,9 +632,17 @@ def main():
     cpu = None
     if not args.no_cpu:
         try:
-            cpu = cpu_baseline(n, side, radii)
+            cpu = cpu_baseline(n, side, radii, recs[S // 2].cpu().numpy(), ions[S // 2].cpu().numpy())
         except Exception as e:  # the baseline is reported, never required
             cpu = {"error": repr(e)}
+    cpu_cfg = None
+    if not args.no_cpu and not args.no_cpu_configs:
+        try:
+            cpu_cfg = cpu_configs(torch, sf, synth)
+        except Exception as e:
+            cpu_cfg = {"error": repr(e)}
+        finally:
+            torch.cuda.empty_cache()
 
     line = {
         "metric": METRIC,
@@ -549,8 +660,9 @@ def main():
         "config": {
             "workload": (f"batched exact bubble finder, {n}^3 float32, side {side:g} Mpc, {S} slices, "
                          f"{len(radii)} radii ({100.0 * n / 512:g} Mpc x 0.9^k to the cell size), "
-                         f"{n_src} sources/slice"),
+                         f"{n_src} sources/slice, emissivity of each slice scaled to mean x_HII ~0.5"),
             "n": n, "slices": S, "radii": len(radii), "radius_batch": K, "tuning": tuned,
+            "slice_zeta": [round(z, 4) for z in zetas], "mean_xhii": [round(1 - v, 4) for v in neu],
         },
         "roofline": roof,
         "step_roofline": step_roof,
@@ -561,6 +673,7 @@ def main():
         "diff": diff,
         "generic_side": generic,
         "cpu_baseline": cpu,
+        "cpu_baseline_configs": cpu_cfg,
     }
     print(json.dumps(line))
 

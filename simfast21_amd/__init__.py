@@ -231,7 +231,8 @@ class HiiContext:
         self.radii = [float(r) for r in radii]
         self._radii = (ctypes.c_double * len(self.radii))(*self.radii)
         cfg = HiiConfig(self.n, float(box_len), len(self.radii), self._radii, input_mode, path,
-                        float(hubble), float(fesc), int(radius_batch), int(device), int(filter_tile), int(filter_w), int(n_halo))
+                        float(hubble), float(fesc), int(radius_batch), int(device), int(filter_tile),
+                        int(filter_w), int(n_halo))
         self._h = ctypes.c_void_p()
         _check(_lib().sf21_hii_create(ctypes.byref(cfg), ctypes.byref(self._h)), "sf21_hii_create")
         self.path = path

@@ -1535,10 +1535,37 @@ int sf21_hii_power_spectrum_dev(sf21_hii_ctx* c, const float* field, int nbins, 
 }
 
 // ---- halo-catalog emissivity (get_HIIbubbles.c:229-259) -----------------
+}  // extern "C"
+
+namespace {
+// Launch constants of the halo-mode emissivity: smooth_box's block size
+// (auxiliary.c:654-656), the volume ratio times fesc (get_HIIbubbles.c:253-258)
+// and Rion's redshift factor (user_functions.c:40-47: c1 (1+z)^c4 / V_norm).
+struct HaloConsts {
+  long sm;
+  float scale, k_rion;
+};
+int halo_consts(const sf21_hii_ctx* c, double z, HaloConsts* k) {
+  const int ng = c->cfg.n_halo, ns = c->n;
+  k->sm = (long)roundf((1.0f * ng) / ns);
+  if ((long)ns * k->sm > ng) k->sm--;
+  if (k->sm < 1) return SF21_ERR_ARG;
+  const double dx_h = c->L / ng;
+  k->scale = (float)(std::pow(c->dx / dx_h, 3) * c->cfg.fesc);
+  k->k_rion = (float)(1.575e39 * std::pow(1.0 + z, 2.27044695e+00) / kVnorm);
+  return SF21_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int sf21_hii_halo_emissivity_dev(sf21_hii_ctx* c, const float* halos, long nhalos, double z,
                                  float* ion_out) {
   if (!c || (!halos && nhalos > 0) || nhalos < 0 || !ion_out) return SF21_ERR_ARG;
   if (c->cfg.input_mode != SF21_INPUT_HALO) return SF21_ERR_ARG;
+  HaloConsts k;
+  if (halo_consts(c, z, &k)) return SF21_ERR_ARG;
   if (hipSetDevice(c->cfg.device) != hipSuccess) return SF21_ERR_HIP;
   (void)hipGetLastError();
   const int ng = c->cfg.n_halo, ns = c->n;
@@ -1546,17 +1573,9 @@ int sf21_hii_halo_emissivity_dev(sf21_hii_ctx* c, const float* halos, long nhalo
     int rc = dev_alloc(&c->halo_grid, (size_t)ng * ng * ng);
     if (rc) return rc;
   }
-  // smooth_box's block size (auxiliary.c:654-656)
-  long sm = (long)roundf((1.0f * ng) / ns);
-  if ((long)ns * sm > ng) sm--;
-  if (sm < 1) return SF21_ERR_ARG;
-  const double dx_h = c->L / ng;
-  const float scale = (float)(std::pow(c->dx / dx_h, 3) * c->cfg.fesc);
-  // Rion (user_functions.c:40-47): c1 (1+z)^c4 / V_norm per redshift
-  const float k_rion = (float)(1.575e39 * std::pow(1.0 + z, 2.27044695e+00) / kVnorm);
   launch(c, "halo_emissivity", [&] {
-    launch_halo_emissivity(nhalos, halos, ng, ns, (int)sm, k_rion, (float)9.367608e+07, 0.44f,
-                           scale, c->halo_grid, ion_out, c->stream);
+    launch_halo_emissivity(nhalos, halos, ng, ns, (int)k.sm, k.k_rion, (float)9.367608e+07, 0.44f,
+                           k.scale, c->halo_grid, ion_out, c->stream);
   });
   if (hipStreamSynchronize(c->stream) != hipSuccess) return SF21_ERR_HIP;
   if (c->prof) collect(c);

@@ -137,4 +137,4 @@ def loss_and_grad(setup, g_np, zeta, tau, cx, cp, nbins=20):
     X, P, _ = relaxed(setup, g, z, tau, nbins)
     L = cx * X + (torch.as_tensor(np.asarray(cp, np.float64)) * P).sum()
     L.backward()
-    return float(L), g.grad.numpy().ravel(), float(z.grad)
+    return float(L.detach()), g.grad.numpy().ravel(), float(z.grad)

@@ -14,7 +14,7 @@ def direct_case(n, zeta=1.0, seed=1001, side=None, rmax=None):
     delta, emis, rec = synth.direct_boxes(n, side, n_src, seed)
     ion = (emis * np.float32(zeta)).astype(np.float32)
     radii = synth.geometric_ladder(rmax, 0.9, side / n)
-    return dict(n=n, side=side, radii=radii, rec=rec, ion=ion, delta=delta)
+    return dict(n=n, side=side, radii=radii, rec=rec, ion=ion, delta=delta, emis=emis)
 
 
 def weights_from_first_centre(fc, radii, dx):

@@ -3,6 +3,7 @@ candidates, it is recorded in the cache file, a new context created with
 automatic settings picks it up from $SF21_TUNE_CACHE, and the tuned context
 still matches the oracle."""
 import os
+import struct
 
 import numpy as np
 import pytest
@@ -10,6 +11,7 @@ import pytest
 import oracle_bind as orc
 import simfast21_amd as sf
 from helpers import direct_case
+from test_gpu_fused import check_vs_oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -26,16 +28,17 @@ def test_autotune_records_and_reloads(torch_cuda, tmp_path):
     t = ctx.tuning()
     print(f"\nautotune {n}^3, {len(radii)} radii: {ms:.3f} ms/slice with {t}", flush=True)
     assert ms > 0
-    assert t["filter_tile"] in (8, 16) and t["filter_w"] in (1, 2) and t["filter_pair"] in (0, 1)
+    # the weight mode is a configuration setting, never a tuned choice
+    assert t["filter_tile"] in (8, 16) and t["filter_w"] == 1 and t["filter_pair"] in (0, 1)
     assert (t["cols_tile"], t["cols_wide"]) in ((8, 0), (16, 0), (16, 2), (16, 1), (16, 3), (32, 1),
                                                 (32, 3))
     assert t["radius_batch"] in (8, 12, 16, len(radii))
     lines = cache.read_text().strip().splitlines()
-    assert len(lines) == 1 and f"n={n} radii={len(radii)}" in lines[0]
+    assert len(lines) == 1 and f"n={n} L={side:.17g} radii={len(radii)} ladder=" in lines[0]
+    assert " w=" not in lines[0]
     x, neu = ctx.run_slice(a, b)
-    xo, neu_o, _ = orc.hii_slice(n, side, radii, case["rec"], case["ion"], precision=64, dilation=1)
-    D = int(np.count_nonzero(np.abs(x.cpu().numpy() - xo) > 1e-4))
-    assert D / n ** 3 < 1e-4 and abs(neu - neu_o) <= max(1e-5, D / n ** 3 + 1e-12)
+    # the tuned context under the full parity bar
+    check_vs_oracle(torch, case, ctx=ctx, label=f"tuned n={n}")
     ctx.close()
     old = os.environ.get("SF21_TUNE_CACHE")
     os.environ["SF21_TUNE_CACHE"] = str(cache)
@@ -56,14 +59,28 @@ def test_autotune_records_and_reloads(torch_cuda, tmp_path):
             os.environ["SF21_TUNE_CACHE"] = old
 
 
-@pytest.mark.parametrize("cols,rows_wide,fp", [((16, 1), 1, 0), ((32, 1), 0, 1), ((8, 0), 1, 0),
-                                               ((16, 2), 0, 1), ((16, 3), 1, 1), ((32, 3), 1, 0)])
-def test_wide_variants_are_bit_identical(torch_cuda, tmp_path, cols, rows_wide, fp):
-    """16 values per thread run the same butterflies on the same values as 8,
-    the prefetching column blocks (cw bit 1) the same transforms on the same
-    tiles, and the filter blocks over mirror line pairs (fp) the same weights
-    (same |k|^2) and transforms: the fused result is identical bit for bit
-    (choices forced through a cache line for this context's key)."""
+def tune_key(arch, n, side, radii):
+    """The cache key sf21_hii_autotune writes: arch, n, box side, ladder
+    length and the FNV-1a hash of the radii's bytes."""
+    h = 1469598103934665603
+    for R in radii:
+        for byte in struct.pack("<d", R):
+            h = ((h ^ byte) * 1099511628211) & (2 ** 64 - 1)
+    return f"{arch} n={n} L={side:.17g} radii={len(radii)} ladder={h:016x}"
+
+
+@pytest.mark.parametrize("cols,rows_wide,fp,tile,K", [
+    ((16, 1), 1, 0, 16, 16), ((32, 1), 0, 1, 16, 16), ((8, 0), 1, 0, 16, 16),
+    ((16, 2), 0, 1, 16, 16), ((16, 3), 1, 1, 16, 16), ((32, 3), 1, 0, 16, 16),
+    ((16, 0), 0, 0, 8, 16), ((16, 0), 0, 1, 8, 12), ((16, 0), 0, 0, 16, 8)])
+def test_wide_variants_are_bit_identical(torch_cuda, tmp_path, cols, rows_wide, fp, tile, K):
+    """Every choice the autotuner searches leaves the result unchanged bit
+    for bit: 16 values per thread run the same butterflies on the same values
+    as 8, the prefetching column blocks (cw bit 1) the same transforms on the
+    same tiles, the filter blocks over mirror line pairs (fp) the same
+    weights (same |k|^2) and transforms, and the filter tile and the radius
+    batch only regroup the same per-line work (choices forced through a
+    cache line for this context's key)."""
     torch = torch_cuda
     case = direct_case(128, zeta=1.0, seed=8200)
     n, side, radii = case["n"], case["side"], case["radii"]
@@ -74,15 +91,15 @@ def test_wide_variants_are_bit_identical(torch_cuda, tmp_path, cols, rows_wide, 
     ref.close()
     arch = torch.cuda.get_device_properties(0).gcnArchName.split(":")[0]
     cache = tmp_path / "tune.txt"
-    cache.write_text(f"{arch} n={n} radii={len(radii)} | tile=16 K=16 w=1 ct={cols[0]} cw={cols[1]} "
+    cache.write_text(f"{tune_key(arch, n, side, radii)} | tile={tile} K={K} ct={cols[0]} cw={cols[1]} "
                      f"rw={rows_wide} fp={fp} | 0.0 ms\n")
     old = os.environ.get("SF21_TUNE_CACHE")
     os.environ["SF21_TUNE_CACHE"] = str(cache)
     try:
         ctx = sf.HiiContext(n, side, radii, sf.INPUT_DIRECT, sf.PATH_FUSED)
         t = ctx.tuning()
-        assert (t["cols_tile"], t["cols_wide"], t["rows_wide"], t["filter_pair"]) == \
-            (cols[0], cols[1], rows_wide, fp)
+        assert (t["cols_tile"], t["cols_wide"], t["rows_wide"], t["filter_pair"],
+                t["filter_tile"], t["radius_batch"]) == (cols[0], cols[1], rows_wide, fp, tile, K)
         x1, neu1 = ctx.run_slice(a, b)
         ctx.close()
     finally:
