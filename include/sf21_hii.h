@@ -129,6 +129,19 @@ int sf21_hii_run_multi_dev(int nbox, sf21_hii_ctx *const *ctx, const float *cons
  * takes host delta and catalog and runs the whole slice. */
 int sf21_hii_halo_emissivity_dev(sf21_hii_ctx *ctx, const float *halos, long n_halos, double z,
                                  float *ion_out);
+/* Allocation-free variant of sf21_hii_halo_emissivity_dev: no counterpart
+ * in the reference; it is for callers that cannot allocate device memory
+ * (code under hipGraph capture, serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and result (bit for bit: the same kernels);
+ * the N_halo^3 deposit grid lives in a library-owned device array of at
+ * most 2^24 floats (64 MiB, N_halo <= 256; larger grids -> SF21_ERR_ARG)
+ * that each GPU holds until the process ends, instead of the context's
+ * lazily allocated grid.  Enqueued on the context's stream without host
+ * synchronisation.  Every call on one GPU shares that GPU's array: calls
+ * that overlap in time on one GPU (other streams, other contexts, other
+ * threads) race. */
+int sf21_hii_halo_emissivity_noalloc(sf21_hii_ctx *ctx, const float *halos, long n_halos,
+                                     double z, float *ion_out);
 int sf21_hii_run_slice_halo(sf21_hii_ctx *ctx, const float *delta, const float *halos,
                             long n_halos, double z, float *xhii, double *neutral);
 

@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1537,6 +1538,18 @@ int sf21_hii_power_spectrum_dev(sf21_hii_ctx* c, const float* field, int nbins, 
 // ---- halo-catalog emissivity (get_HIIbubbles.c:229-259) -----------------
 }  // extern "C"
 
+// Scratch of sf21_hii_halo_emissivity_noalloc: the N_halo^3 CIC deposit
+// grid, at most kNoAllocGrid floats (64 MiB: N_halo <= 256), in a
+// file-scope device array that every GPU the library runs on holds from
+// module load until the process ends.  In host code the array's name is a
+// host-side stand-in, not a device address: each device's address comes
+// from hipGetSymbolAddress, taken once per device (with that device
+// current) on the entry point's first call there, and only that address is
+// passed to kernels.  Calls on one GPU share its array: each call zeroes the
+// grid before depositing, and calls that overlap on one GPU race.
+constexpr long kNoAllocGrid = 1L << 24;
+static __device__ __align__(256) float scratch[kNoAllocGrid];
+
 namespace {
 // Launch constants of the halo-mode emissivity: smooth_box's block size
 // (auxiliary.c:654-656), the volume ratio times fesc (get_HIIbubbles.c:253-258)
@@ -1556,6 +1569,26 @@ int halo_consts(const sf21_hii_ctx* c, double z, HaloConsts* k) {
   return SF21_OK;
 }
 
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float* g_scratch_dev[kMaxDevices] = {};
+
+// The device address of this device's scratch array (the device must be
+// current).
+int scratch_address(int device, float** out) {
+  if (device < 0 || device >= kMaxDevices) return SF21_ERR_ARG;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[device]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+      (void)hipGetLastError();
+      return SF21_ERR_HIP;
+    }
+    g_scratch_dev[device] = static_cast<float*>(p);
+  }
+  *out = g_scratch_dev[device];
+  return SF21_OK;
+}
 }  // namespace
 
 extern "C" {
@@ -1579,6 +1612,25 @@ int sf21_hii_halo_emissivity_dev(sf21_hii_ctx* c, const float* halos, long nhalo
   });
   if (hipStreamSynchronize(c->stream) != hipSuccess) return SF21_ERR_HIP;
   if (c->prof) collect(c);
+  return check_hip();
+}
+
+int sf21_hii_halo_emissivity_noalloc(sf21_hii_ctx* c, const float* halos, long nhalos, double z,
+                                     float* ion_out) {
+  if (!c || (!halos && nhalos > 0) || nhalos < 0 || !ion_out) return SF21_ERR_ARG;
+  if (c->cfg.input_mode != SF21_INPUT_HALO) return SF21_ERR_ARG;
+  const long ng = c->cfg.n_halo;
+  if (ng * ng * ng > kNoAllocGrid) return SF21_ERR_ARG;
+  HaloConsts k;
+  if (halo_consts(c, z, &k)) return SF21_ERR_ARG;
+  if (hipSetDevice(c->cfg.device) != hipSuccess) return SF21_ERR_HIP;
+  (void)hipGetLastError();
+  float* grid = nullptr;
+  int rc = scratch_address(c->cfg.device, &grid);
+  if (rc) return rc;
+  // enqueued on the context's stream, no host synchronisation (capturable)
+  launch_halo_emissivity(nhalos, halos, (int)ng, c->n, (int)k.sm, k.k_rion, (float)9.367608e+07,
+                         0.44f, k.scale, grid, ion_out, c->stream);
   return check_hip();
 }
 
