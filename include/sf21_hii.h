@@ -252,8 +252,8 @@ int sf21_hii_filtered_dev(sf21_hii_ctx *ctx, const float *a, const float *b, con
 /* Autotuner (fused path; no counterpart in the reference, whose FFTW plans
  * are tuned by FFTW_MEASURE at get_HIIbubbles.c:110-183).  Times the slice
  * (a, b, c, z; device pointers) by a coordinate search over launch shapes
- * and stage orders: radius batch 16/12/8 (capped by the ladder; skipped
- * when it does not fit), filter tile 16/8, filter blocks over single
+ * and stage orders: radius batch 16, 8, 12, 24, 32 or the whole ladder
+ * (capped by the ladder; skipped when it does not fit), filter tile 16/8, filter blocks over single
  * i-lines or mirror pairs (kj, n - kj) sharing W, the per-radius column
  * passes (8 values per thread with 8 or 16 columns per block, or 16 values
  * per thread with 16 or 32 columns, the latter for n = 128, 256, 512; with

@@ -1425,7 +1425,9 @@ int sf21_hii_autotune(sf21_hii_ctx* c, const float* a, const float* b, const flo
       best = t;
     }
   };
-  for (int K : {8, 12}) {
+  // larger batches read the spectrum fewer times per slice; candidates that
+  // do not fit in device memory are skipped
+  for (int K : {8, 12, 24, 32, nr}) {
     TuneChoice t = best;
     t.K = std::min(K, nr);
     if (t.K != best.K) consider(t);

@@ -32,7 +32,7 @@ def test_autotune_records_and_reloads(torch_cuda, tmp_path):
     assert t["filter_tile"] in (8, 16) and t["filter_w"] == 1 and t["filter_pair"] in (0, 1)
     assert (t["cols_tile"], t["cols_wide"]) in ((8, 0), (16, 0), (16, 2), (16, 1), (16, 3), (32, 1),
                                                 (32, 3))
-    assert t["radius_batch"] in (8, 12, 16, len(radii))
+    assert t["radius_batch"] in (8, 12, 16, 24, 32, len(radii))
     lines = cache.read_text().strip().splitlines()
     assert len(lines) == 1 and f"n={n} L={side:.17g} radii={len(radii)} ladder=" in lines[0]
     assert " w=" not in lines[0]
