@@ -396,9 +396,9 @@ __global__ void k_fill_i32(int* __restrict__ p, int value, long count) {
 // in LDS.  On the unrolled line, h(x, y) = f(y) - (x-y)^2 is supermodular
 // (h(x1,y1) + h(x2,y2) - h(x1,y2) - h(x2,y1) = 2(x2-x1)(y2-y1) > 0), so the
 // maximisers of x are non-decreasing in x.  Outputs are found in three
-// levels: every 8*Q-th x by an outward scan with early exit (stop once
-// d^2 > fmax - best), every Q-th x inside the bracket of its level-0
-// neighbours, the rest inside the bracket of their level-1 neighbours.  A
+// levels: every 8*Q-th x over its whole window (8 lanes splitting it), every
+// Q-th x inside the bracket of its level-0 neighbours, the rest inside the
+// bracket of their level-1 neighbours.  A
 // neighbour xa that found no non-negative value bounds the search too: a
 // y < xa with f(y) >= (x-y)^2 > (xa-y)^2 would contradict it.  Only values
 // >= 0 are ever needed, so candidates farther than floor(sqrt(fmax)) are
@@ -423,17 +423,6 @@ __device__ __forceinline__ void consider(int x, int y, F f, Found& r) {
     r.lo = min(r.lo, y);
     r.hi = max(r.hi, y);
   }
-}
-
-template <class F>
-__device__ __forceinline__ Found scan_out(int x, int hl, int fmax, F f) {
-  Found r{-1, 0, 0};
-  for (int d = 0; d <= hl; ++d) {
-    if (r.best >= 0 && d * d > fmax - r.best) break;
-    consider(x, x - d, f, r);
-    if (d) consider(x, x + d, f, r);
-  }
-  return r;
 }
 
 template <class F>
@@ -480,9 +469,31 @@ __device__ __forceinline__ void dt_line_wave(const int* col, int n, int Q, int h
   }
   const int hl = min(hcap, isqrt_floor(fmax));
   const int x1 = Q * lane;
-  // level 0: lanes 0, 8, 16, ... (x = 8*Q*m)
+  // level 0: the points x = 8*Q*m, one per group of 8 lanes (8m .. 8m + 7):
+  // lane k of the group takes the candidates x + d, d = -hl + k (mod 8), of
+  // the window [x - hl, x + hl], then a 3-step butterfly leaves the group's
+  // (best, lo, hi) in every lane of it
   Found l0{-1, 0, 0};
-  if (act && (lane & 7) == 0) l0 = scan_out(x1, hl, fmax, f);
+  {
+    const int g0 = lane & ~7;
+    if (g0 < P1) {
+      const int x0 = Q * g0;
+      for (int d = -hl + (lane & 7); d <= hl; d += 8) consider(x0, x0 + d, f, l0);
+    }
+#pragma unroll
+    for (int o = 1; o < 8; o <<= 1) {
+      const int ob = __shfl_xor(l0.best, o, 64);
+      const int olo = __shfl_xor(l0.lo, o, 64), ohi = __shfl_xor(l0.hi, o, 64);
+      if (ob > l0.best) {
+        l0.best = ob;
+        l0.lo = olo;
+        l0.hi = ohi;
+      } else if (ob == l0.best && ob >= 0) {
+        l0.lo = min(l0.lo, olo);
+        l0.hi = max(l0.hi, ohi);
+      }
+    }
+  }
   // level 1: bracket [xa, xb] of level-0 points, xb = xa + 8Q (wrapping to 0 + n)
   const int ga = lane & ~7;
   const int gb = ga + 8;
@@ -508,6 +519,60 @@ __device__ __forceinline__ void dt_line_wave(const int* col, int n, int Q, int h
       const int hi = n_best >= 0 ? n_hi0 + (lane + 1 < P1 ? 0 : n) : x1 + Q;
       out[q] = scan_range(x1 + q, lo, hi, hl, f).best;
     }
+  }
+}
+
+// Coverage only (g(x) >= 0) of one line held in LDS, for the last axis.
+// Cell x is covered iff some y with f(y) >= 0 lies within r(y) = min(hcap,
+// floor(sqrt(f(y)))) of it on the ring: d^2 <= f(y) <=> |d| <= floor(sqrt
+// f(y)), and the transform above never looks past hl <= hcap.  As r <= n/2,
+// the images of y that can reach x in [0, n) are y - n, y and y + n, so x is
+// covered from the left iff max(max_{y <= x} (y + r), max_y (y + r) - n) >= x
+// and from the right iff min(min_{y >= x} (y - r), min_y (y - r) + n) <= x:
+// one prefix max and one suffix min over the line (in-lane over the lane's Q
+// cells, then across the wave).  out[q] = 1 for covered cells x = Q*lane + q.
+template <int QMAX>
+__device__ __forceinline__ void cover_line_wave(const int* col, int n, int Q, int hcap, int lane,
+                                                int (&out)[QMAX]) {
+  constexpr int kNone = 1 << 29;
+  const bool act = lane < n / Q;
+  int rr[QMAX], ll[QMAX];
+#pragma unroll
+  for (int q = 0; q < QMAX; ++q) {
+    rr[q] = -kNone;
+    ll[q] = kNone;
+    if (act && q < Q) {
+      const int y = Q * lane + q;
+      const int v = col[y + (y >> 3)];
+      if (v >= 0) {
+        const int r = min(hcap, isqrt_floor(v));
+        rr[q] = y + r;
+        ll[q] = y - r;
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 1; q < QMAX; ++q) rr[q] = max(rr[q], rr[q - 1]);
+#pragma unroll
+  for (int q = QMAX - 2; q >= 0; --q) ll[q] = min(ll[q], ll[q + 1]);
+  // inclusive scans of the lane totals: max from lane 0 up, min from lane 63 down
+  int sr = rr[QMAX - 1], sl = ll[0];
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int ur = __shfl_up(sr, o, 64);
+    const int dl = __shfl_down(sl, o, 64);
+    if (lane >= o) sr = max(sr, ur);
+    if (lane + o < 64) sl = min(sl, dl);
+  }
+  const int rall = __shfl(sr, 63, 64), lall = __shfl(sl, 0, 64);
+  int er = __shfl_up(sr, 1, 64), el = __shfl_down(sl, 1, 64);
+  if (lane == 0) er = -kNone;
+  if (lane == 63) el = kNone;
+  const int cr = max(er, rall - n), cl = min(el, lall + n);
+#pragma unroll
+  for (int q = 0; q < QMAX; ++q) {
+    const int x = Q * lane + q;
+    out[q] = (max(rr[q], cr) >= x || min(ll[q], cl) <= x) ? 1 : 0;
   }
 }
 
@@ -587,15 +652,15 @@ __global__ __launch_bounds__(256) void k_dt_rows_final(DtArgs a, const int* in,
     lds_pin();
     __syncthreads();
     int o[QMAX];
-    if (valid) dt_line_wave<QMAX>(col, n, Q, a.h, lane, o);
+    if (valid) cover_line_wave<QMAX>(col, n, Q, a.h, lane, o);
     if (valid) {
-      // the lane's Q consecutive results go back into the wave's own column
-      // (the transform's reads of it are complete), then lane takes cells
+      // the lane's Q consecutive flags go back into the wave's own column
+      // (the line's reads of it are complete), then lane takes cells
       // lane, lane + 64, ...: coalesced loads of the fields and stores of x
       if (lane < n / Q)
         for (int q = 0; q < Q; ++q) {
           const int e = Q * lane + q;
-          col[e + (e >> 3)] = o[q] >= 0 ? 1 : 0;
+          col[e + (e >> 3)] = o[q];
         }
       lds_pin();
       __builtin_amdgcn_wave_barrier();
