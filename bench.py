@@ -352,6 +352,9 @@ def bench_generic(torch, sf, synth, n=600, steps=2, seed=6006):
     torch.cuda.empty_cache()
     ctx = sf.HiiContext(n, side, radii, sf.INPUT_DIRECT, sf.PATH_FUSED)
     out = torch.empty_like(recs[0])
+    # the line passes' XCD block order (results identical for every choice)
+    tune_ms = ctx.autotune(recs[0], ions[0])
+    order = ctx.line_order
     ctx.run_slice(recs[0], ions[0], xhii=out)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -376,6 +379,7 @@ def bench_generic(torch, sf, synth, n=600, steps=2, seed=6006):
         kern[k] = e
     return {"n": n, "side": side, "radii": len(radii), "ms_per_slice": round(ms, 2),
             "voxel_radius_per_s": v * len(radii) / (ms * 1e-3), "mean_xhii": round(1 - neu, 6),
+            "line_order": order, "tune_ms_per_slice": round(tune_ms, 2),
             "algorithmic_bytes_per_voxel_radius": 40, "kernels": kern}
 
 

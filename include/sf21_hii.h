@@ -63,8 +63,9 @@ typedef struct sf21_hii_config {
   int n;                 /* N_smoothed: 64, 96, 128, 192, 256, 512 (32 fused only); fused
                             path also any other multiple of 8 in 16..768 with factors
                             2, 3, 5 and n/q a multiple of 8, at most 64, for some q <= 16
-                            (e.g. 80, 120, 200, 360, 600); those sides refuse the
-                            autotuner and the differentiable mode */
+                            (e.g. 80, 120, 200, 360, 600); there the autotuner
+                            searches the line passes' block order and the
+                            differentiable mode stores no fields */
   double box_len;        /* box side L (unit of the radii) */
   int n_radii;           /* ladder length */
   const double *radii;   /* descending ladder (get_HIIbubbles.c:298-461), every radius > 0
@@ -190,7 +191,10 @@ int sf21_nldensity_dev(sf21_hii_ctx *ctx, const sf21_params *p, double z, const 
 /* ---- Differentiable mode -------------------------------------------------
  * No counterpart in the reference (SimFast21 has no gradients); it wraps the
  * bubble finder of get_HIIbubbles.c:277-479 for inference work.  Context:
- * input_mode SF21_INPUT_DIRECT, path SF21_PATH_FUSED, n >= 64.  Inputs
+ * input_mode SF21_INPUT_DIRECT, path SF21_PATH_FUSED, n >= 64; sides
+ * without a register plan (such as 600) run the same model through their
+ * line transforms, one radius pair at a time, always recomputing the
+ * per-radius fields.  Inputs
  * (device, n^3 floats): g, a Gaussian field, and emis, an emissivity
  * pattern; zeta scales it.  The model:
  *   Delta = exp(g)/<exp(g)>, rec = Delta^2, ion = zeta*emis, no residual
@@ -221,7 +225,9 @@ int sf21_nldensity_dev(sf21_hii_ctx *ctx, const sf21_params *p, double z, const 
  * changed in place between the two calls, values swapped between cells
  * included, is detected except with probability ~2^-64); otherwise it
  * recomputes as in mode 1.  The gradient is identical in every mode;
- * SF21_ERR_ALLOC if mode 1 or 3 does not fit. */
+ * SF21_ERR_ALLOC if mode 1 or 3 does not fit.  Sides without a register
+ * plan accept modes 0 and 2 only (SF21_ERR_ARG otherwise) and store
+ * nothing. */
 int sf21_hii_diff_store_fields(sf21_hii_ctx *ctx, int mode);
 /* 1 if the last diff_backward read fields kept by the forward (mode 3), 0
  * otherwise. */
@@ -265,7 +271,11 @@ int sf21_hii_filtered_dev(sf21_hii_ctx *ctx, const float *a, const float *b, con
  * (device arch, n, box_len, the radii).  A context created with
  * radius_batch, filter_tile and filter_w all 0 takes the cached choice for
  * its key from $SF21_TUNE_CACHE.  *best_ms (nullable): the winner's time per
- * slice. */
+ * slice.  Sides without a register plan (one radius per step through line
+ * transforms) have one choice: which column passes (along j, along i) deal
+ * runs of consecutive blocks to one XCD (sf21_hii_line_order); the four
+ * masks are timed and the winner is cached the same way.  Every choice
+ * changes launch shapes or block order only: the results are identical. */
 int sf21_hii_autotune(sf21_hii_ctx *ctx, const float *a, const float *b, const float *c,
                       double z, const char *cache_path, double *best_ms);
 /* Current choices: filter tile, radius batch, top-hat weights (1 in the
@@ -279,6 +289,11 @@ int sf21_hii_tuning(const sf21_hii_ctx *ctx, int *filter_tile, int *radius_batch
 /* Effective radii per fused filter pass (the automatic choice depends on
  * free device memory); negative on error. */
 int sf21_hii_radius_batch(const sf21_hii_ctx *ctx);
+/* Sides without a register plan: the XCD-order mask of the line passes
+ * (bit 1: passes along j, bit 2: along i; default 4); SF21_ERR_ARG on other
+ * sides.  No counterpart in the reference (a launch choice of this
+ * implementation). */
+int sf21_hii_line_order(const sf21_hii_ctx *ctx);
 
 /* Kernel timing with HIP events on the context's stream. */
 int sf21_hii_profile(sf21_hii_ctx *ctx, int enable);

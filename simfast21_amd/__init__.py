@@ -123,6 +123,7 @@ def _lib():
     L.sf21_hii_filtered_dev.argtypes = [vp, vp, vp, vp, d, i, vp]
     L.sf21_hii_profile.argtypes = [vp, i]
     L.sf21_hii_radius_batch.argtypes = [vp]
+    L.sf21_hii_line_order.argtypes = [vp]
     L.sf21_hii_autotune.argtypes = [vp, vp, vp, vp, d, ctypes.c_char_p, dp]
     L.sf21_hii_tuning.argtypes = [vp, ip, ip, ip, ip, ip, ip, ip]
     L.sf21_hii_kernel_stats.argtypes = [vp, i, ctypes.c_char_p, i, ctypes.POINTER(ctypes.c_long), dp]
@@ -438,6 +439,14 @@ class HiiContext:
     @property
     def radius_batch(self) -> int:
         return int(_lib().sf21_hii_radius_batch(self._h))
+
+    @property
+    def line_order(self) -> int:
+        """Sides without a register plan: the XCD-order mask of the line
+        passes (bit 1: along j, bit 2: along i)."""
+        v = int(_lib().sf21_hii_line_order(self._h))
+        _check(min(v, 0), "sf21_hii_line_order")
+        return v
 
     def profile(self, enable: bool = True):
         _check(_lib().sf21_hii_profile(self._h, int(enable)), "sf21_hii_profile")

@@ -28,6 +28,8 @@ struct GenPlan {
   int L;
   int nst;
   int lp;          // LDS line stride (float2)
+  int xcd;         // bit a (a = 1: j, 2: i): the passes along axis a deal runs of
+                   // consecutive blocks to one XCD (an autotuner choice)
   int radix[16];
   int ns[16];
   float rns[16];   // 1 / ns in FP32 (exact quotients of the lane indices)
@@ -176,6 +178,17 @@ void launch_grad_cells(long n3, const float2* gacc, const float* qbar, const flo
                        hipStream_t s);
 void launch_grad_g(long n3, const float* delta1, const float* dd, const double* st, float* gout,
                    hipStream_t s);
+// generic sides: full n^3 complex boxes, transforms by launch_gen_pass
+int gen_seed_blocks(long n3);  // rho accumulators written by launch_gen_seed
+void launch_gen_phi(long n3, const float2* pair, float2* slot, int half, const float* bal,
+                    const double* st, float inv_tau, hipStream_t s);
+void launch_gen_ball(int n, bool acc, const float2* in, float2* out, const float* ta,
+                     const float* tb, hipStream_t s);
+void launch_gen_seed(long n3, float2* u, const float2* v, int half, const float* bal,
+                     const double* st, float inv_tau, double* rho_acc, hipStream_t s);
+void launch_gen_filter_acc(int n, const float2* in, float2* acc, float2 kappa, hipStream_t s);
+void launch_pk_adjoint_full(int n, float2* spec, const short* bin_of_m, const double* coef,
+                            hipStream_t s);
 void launch_fill_c(float2* p, long count, hipStream_t s);
 void launch_fill_d(double* p, long count, hipStream_t s);
 void launch_to_complex(long n3, const float* in, float2* out, hipStream_t s);

@@ -256,6 +256,36 @@ __device__ __forceinline__ float rho_u_of(const double* st) {
   return (float)(1e-3 * st[DS_SUM_REC]);
 }
 
+// phi_R of one cell from its pair in unnormalised units (ru = N R_R, iu =
+// N I_R)
+__device__ __forceinline__ float phi_cell(float ru, float iu, float rho_u, float inv_tau) {
+  return phi_of(fminf(iu - ru, iu) / fmaxf(ru, rho_u) * inv_tau);
+}
+
+// Seed of one cell (the adjoints of the normalised fields, Rbar + i Ibar)
+// from its pair (ru, iu) and N phibar:
+//   sbar = phibar phi'(s/tau)/tau,  phi'(u) = sigmoid(u) - sigmoid(u - c),
+//   Rbar = sbar ds/dR,  Ibar = sbar ds/dI,
+//   ds/dI = 1/den, ds/dR = -[R >= 0]/den - [R > rho] num/den^2,
+//   num = min(I - R, I), den = max(R, rho);
+// the rho term sbar ds/drho = -sbar [R <= rho] num/rho^2 is added to rsum.
+// In unnormalised units the factors N of ds/d(.) and of phibar cancel.
+__device__ __forceinline__ float2 seed_cell(float ru, float iu, float phib, float rho_u,
+                                            float inv_tau, double& rsum) {
+  const float na = iu - ru;
+  const bool use_a = na <= iu;
+  const float num = use_a ? na : iu;
+  const bool den_r = ru > rho_u;
+  const float den = den_r ? ru : rho_u;
+  const float inv_den = 1.0f / den;
+  const float s = num * inv_den;
+  const float sb = phib * dphi_of(s * inv_tau) * inv_tau;
+  const float q = num * inv_den * inv_den;
+  const float dR = (use_a ? -inv_den : 0.0f) - (den_r ? q : 0.0f);
+  if (!den_r) rsum -= (double)sb * (double)q;
+  return make_float2(sb * dR, sb * inv_den);
+}
+
 // ---------------------------------------------------------------------------
 // Sweep 1, rows: inverse along p of each radius' pair, phi_R, pairs packed
 // and transformed forward along p, written in place to slot r/2 (slot r/2's
@@ -303,10 +333,7 @@ __global__ __launch_bounds__(256) void k_rows_phi(float2* u, int nrad, long ustr
     }
     float ph[E];
 #pragma unroll
-    for (int m = 0; m < E; ++m) {
-      const float ru = v[m].x, iu = v[m].y * inv_s;
-      ph[m] = phi_of(fminf(iu - ru, iu) / fmaxf(ru, rho_u) * inv_tau);
-    }
+    for (int m = 0; m < E; ++m) ph[m] = phi_cell(v[m].x, v[m].y * inv_s, rho_u, inv_tau);
     if ((r & 1) == 0 && r + 1 < nrad) {
 #pragma unroll
       for (int m = 0; m < E; ++m) pa[m] = ph[m];
@@ -398,15 +425,9 @@ __global__ __launch_bounds__(TILE*(L / Plan<L>::E)) void k_ball_i(
 
 // ---------------------------------------------------------------------------
 // Sweep 2, rows: inverse along p of the recomputed pair (N R_R, s N I_R) and
-// of the ball pair (N phibar_a, N phibar_b); seeds of the normalised fields
-//   sbar = phibar phi'(s/tau)/tau,  phi'(u) = sigmoid(u) - sigmoid(u - c),
-//   Rbar = sbar ds/dR,  Ibar = sbar ds/dI,
-//   ds/dI = 1/den, ds/dR = -[R >= 0]/den - [R > rho] num/den^2,
-//   num = min(I - R, I), den = max(R, rho)
-// packed Rbar + i Ibar, forward along p, in place over the pair.  The rho
-// term sbar ds/drho = -sbar [R <= rho] num/rho^2 is summed in FP64 into
-// rho_acc[block] (accumulated over the batches).  In unnormalised units the
-// factors N of ds/d(.) and of phibar cancel.
+// of the ball pair (N phibar_a, N phibar_b); the seeds (seed_cell) packed
+// Rbar + i Ibar, forward along p, in place over the pair.  The rho terms are
+// summed in FP64 into rho_acc[block] (accumulated over the batches).
 // ---------------------------------------------------------------------------
 template <int L, bool STORED>
 __global__ __launch_bounds__(256) void k_rows_seed(float2* u, const float2* v, int nrad,
@@ -454,22 +475,8 @@ __global__ __launch_bounds__(256) void k_rows_seed(float2* u, const float2* v, i
         line_fft<L, +1>(y, sre, sim, tl, tw);
       }
 #pragma unroll
-      for (int m = 0; m < E; ++m) {
-        const float ru = y[m].x, iu = y[m].y * inv_s;
-        const float na = iu - ru;
-        const bool use_a = na <= iu;
-        const float num = use_a ? na : iu;
-        const bool den_r = ru > rho_u;
-        const float den = den_r ? ru : rho_u;
-        const float inv_den = 1.0f / den;
-        const float s = num * inv_den;
-        const float phib = hh ? pb[m].y : pb[m].x;
-        const float sb = phib * dphi_of(s * inv_tau) * inv_tau;
-        const float q = num * inv_den * inv_den;
-        const float dR = (use_a ? -inv_den : 0.0f) - (den_r ? q : 0.0f);
-        y[m] = make_float2(sb * dR, sb * inv_den);
-        if (!den_r) rsum -= (double)sb * (double)q;
-      }
+      for (int m = 0; m < E; ++m)
+        y[m] = seed_cell(y[m].x, y[m].y * inv_s, hh ? pb[m].y : pb[m].x, rho_u, inv_tau, rsum);
       line_fft<L, -1>(y, sre, sim, tl, tw);
 #pragma unroll
       for (int m = 0; m < E; ++m) src[tl + TPL * m] = y[m];
@@ -536,6 +543,107 @@ __global__ __launch_bounds__(TILE*(L / Plan<L>::E)) void k_filter_acc_i(
   for (int m = 0; m < E; ++m) {
     const float2 o = dst[(t + TPL * m) * (L * L)];
     dst[(t + TPL * m) * (L * L)] = make_float2(o.x + z[m].x, o.y + z[m].y);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Generic sides (no register plan): the per-radius steps of both sweeps as
+// passes over full n^3 complex boxes; the 3D transforms between them run as
+// separate line passes (k_gen_pass), and one radius pair shares a complex
+// transform as on the plan sides.
+// ---------------------------------------------------------------------------
+// phi of one radius from its real-space pair (N R_R, s N I_R) into the real
+// (half 0: imaginary part cleared) or imaginary part (half 1) of slot.
+__global__ void k_gen_phi(long n3, const float2* __restrict__ pair, float2* __restrict__ slot,
+                          int half, const float* __restrict__ bal, const double* __restrict__ st,
+                          float inv_tau) {
+  const float inv_s = 1.0f / bal[0];
+  const float rho_u = rho_u_of(st);
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n3;
+       i += (long)gridDim.x * blockDim.x) {
+    const float2 v = pair[i];
+    const float ph = phi_cell(v.x, v.y * inv_s, rho_u, inv_tau);
+    slot[i] = half ? make_float2(slot[i].x, ph) : make_float2(ph, 0.0f);
+  }
+}
+
+// Ball products on the full spectrum (tb null: the pair's second radius is
+// absent).  ACC (sweep 1): out += (B_a - i B_b) in; otherwise (sweep 2):
+// out = (B_a + i B_b) in -- the products of k_ball_i.
+template <bool ACC>
+__global__ void k_gen_ball(int n, const float2* __restrict__ in, float2* __restrict__ out,
+                           const float* __restrict__ ta, const float* __restrict__ tb) {
+  const long n3 = (long)n * n * n;
+  const int H = n / 2 + 1;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n3;
+       i += (long)gridDim.x * blockDim.x) {
+    const int p = (int)(i % n), j = (int)((i / n) % n), a = (int)(i / ((long)n * n));
+    const long ti = ((long)fold_idx(a, n) * H + fold_idx(j, n)) * H + fold_idx(p, n);
+    const float ba = ta[ti], bb = tb ? tb[ti] : 0.0f;
+    const float2 v = in[i];
+    if constexpr (ACC) {
+      const float2 o = out[i];
+      out[i] = make_float2(o.x + (ba * v.x + bb * v.y), o.y + (ba * v.y - bb * v.x));
+    } else {
+      out[i] = make_float2(ba * v.x - bb * v.y, ba * v.y + bb * v.x);
+    }
+  }
+}
+
+// Seeds of one radius in place over its real-space pair; phibar from the real
+// (half 0) or imaginary part (half 1) of v = N (phibar_a + i phibar_b); the
+// rho terms summed in FP64 into rho_acc[block].
+__global__ __launch_bounds__(256) void k_gen_seed(long n3, float2* __restrict__ u,
+                                                  const float2* __restrict__ v, int half,
+                                                  const float* __restrict__ bal,
+                                                  const double* __restrict__ st, float inv_tau,
+                                                  double* __restrict__ rho_acc) {
+  __shared__ double r1[256], r2[256];
+  const float inv_s = 1.0f / bal[0];
+  const float rho_u = rho_u_of(st);
+  double rsum = 0.0;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n3;
+       i += (long)gridDim.x * blockDim.x) {
+    const float2 y = u[i], b = v[i];
+    u[i] = seed_cell(y.x, y.y * inv_s, half ? b.y : b.x, rho_u, inv_tau, rsum);
+  }
+  block_reduce2(rsum, 0.0, r1, r2);
+  if (threadIdx.x == 0) rho_acc[blockIdx.x] += r1[0];
+}
+
+// acc += W_R(|k|) in on the full spectrum: the weight of the forward filter
+// pass (k_gen_pass mode 1), the same function of the same |k|^2.
+__global__ void k_gen_filter_acc(int n, const float2* __restrict__ in, float2* __restrict__ acc,
+                                 float2 kappa) {
+  const long n3 = (long)n * n * n;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n3;
+       i += (long)gridDim.x * blockDim.x) {
+    const int p = (int)(i % n), j = (int)((i / n) % n), a = (int)(i / ((long)n * n));
+    const float w = w_tophat_k(kappa, fold_sq(a, n) + fold_sq(j, n) + fold_sq(p, n));
+    const float2 v = in[i], o = acc[i];
+    acc[i] = make_float2(o.x + w * v.x, o.y + w * v.y);
+  }
+}
+
+// P(k) adjoint on the full spectrum F of T (the binned modes are the p <=
+// n/2 half): H = 2 c_b F there, 0 elsewhere; the real part of the
+// unnormalised inverse of H is dL/dT = sum_half 2 c_b Re(F e^{ikx}).
+__global__ void k_pk_adjoint_full(int n, float2* __restrict__ spec,
+                                  const short* __restrict__ bin_of_m,
+                                  const double* __restrict__ coef) {
+  const long n3 = (long)n * n * n;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n3;
+       i += (long)gridDim.x * blockDim.x) {
+    const int p = (int)(i % n);
+    const long r = i / n;
+    float c = 0.0f;
+    if (p <= n / 2) {
+      const int m = fold_sq((int)(r / n), n) + fold_sq((int)(r % n), n) + p * p;
+      const int b = bin_of_m[m];
+      c = b >= 0 ? 2.0f * (float)coef[b] : 0.0f;
+    }
+    const float2 v = spec[i];
+    spec[i] = make_float2(v.x * c, v.y * c);
   }
 }
 
@@ -890,6 +998,33 @@ void launch_fill_d(double* p, long count, hipStream_t s) {
 }
 void launch_to_complex(long n3, const float* in, float2* out, hipStream_t s) {
   hipLaunchKernelGGL(k_to_complex, dim3(eb(n3)), dim3(256), 0, s, n3, in, out);
+}
+
+int gen_seed_blocks(long n3) { return (int)eb(n3); }
+void launch_gen_phi(long n3, const float2* pair, float2* slot, int half, const float* bal,
+                    const double* st, float inv_tau, hipStream_t s) {
+  hipLaunchKernelGGL(k_gen_phi, dim3(eb(n3)), dim3(256), 0, s, n3, pair, slot, half, bal, st,
+                     inv_tau);
+}
+void launch_gen_ball(int n, bool acc, const float2* in, float2* out, const float* ta,
+                     const float* tb, hipStream_t s) {
+  const long n3 = (long)n * n * n;
+  if (acc) hipLaunchKernelGGL(k_gen_ball<true>, dim3(eb(n3)), dim3(256), 0, s, n, in, out, ta, tb);
+  else hipLaunchKernelGGL(k_gen_ball<false>, dim3(eb(n3)), dim3(256), 0, s, n, in, out, ta, tb);
+}
+void launch_gen_seed(long n3, float2* u, const float2* v, int half, const float* bal,
+                     const double* st, float inv_tau, double* rho_acc, hipStream_t s) {
+  hipLaunchKernelGGL(k_gen_seed, dim3(eb(n3)), dim3(256), 0, s, n3, u, v, half, bal, st, inv_tau,
+                     rho_acc);
+}
+void launch_gen_filter_acc(int n, const float2* in, float2* acc, float2 kappa, hipStream_t s) {
+  const long n3 = (long)n * n * n;
+  hipLaunchKernelGGL(k_gen_filter_acc, dim3(eb(n3)), dim3(256), 0, s, n, in, acc, kappa);
+}
+void launch_pk_adjoint_full(int n, float2* spec, const short* bin_of_m, const double* coef,
+                            hipStream_t s) {
+  const long n3 = (long)n * n * n;
+  hipLaunchKernelGGL(k_pk_adjoint_full, dim3(eb(n3)), dim3(256), 0, s, n, spec, bin_of_m, coef);
 }
 
 }  // namespace sf21

@@ -216,12 +216,13 @@ __global__ __launch_bounds__(256, 4) void k_gen_pass(const float2* src, float2* 
                                                   int* __restrict__ wmap, int d2) {
   extern __shared__ float2 gsm[];
   const unsigned L = LC ? (unsigned)LC : (unsigned)plan.L, L2 = L * L;
-  // i-axis passes: consecutive tiles (which share 128 B lines) on one XCD --
-  // workgroups are dealt round-robin over the 8 XCDs, and the grid (L^2/8
-  // blocks, L a multiple of 8) is a multiple of 8.  Measured at 600: the
-  // filtered i pass 1.30 -> 1.15 ms; the j pass 0.80 -> 0.81 ms, so it keeps
-  // the plain order.
-  const unsigned blk = (!ROWS && axis == 2)
+  // column passes with bit `axis` of plan.xcd: consecutive tiles (which
+  // share 128 B lines) on one XCD -- workgroups are dealt round-robin over
+  // the 8 XCDs, and the grid (L^2/8 blocks, L a multiple of 8) is a multiple
+  // of 8, so the remap is a permutation.  Measured at 600: the filtered i
+  // pass 1.30 -> 1.15 ms, the j pass 0.80 -> 0.81 ms (default: i passes
+  // only; the autotuner searches the four masks).
+  const unsigned blk = (!ROWS && ((plan.xcd >> axis) & 1))
                            ? (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3)
                            : blockIdx.x;
   const int l = ROWS ? (int)(threadIdx.x >> 5) : (int)(threadIdx.x & 7);
@@ -337,6 +338,7 @@ bool gen_plan_make(int n, GenPlan* p) {
   }
   p->L = n;
   p->nst = st;
+  p->xcd = 1 << 2;
   if (p->radix[0] != 8 || st < 2) return false;  // the kernels' first stage is radix 8
   // line stride = 8 (mod 32) float2: the 8 lines of a column-pass wave
   // start in different bank groups

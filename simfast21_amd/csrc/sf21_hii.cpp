@@ -564,6 +564,16 @@ void enqueue_pk(sf21_hii_ctx* c, const float* field, float2* half, int nbins) {
 }
 
 // ---- differentiable mode ----------------------------------------------
+// Unnormalised 3D complex transform in place on a generic side (line passes
+// along p, j, i).
+void gen_fft3(sf21_hii_ctx* c, float2* x, int sign) {
+  const float2 none = make_float2(0.f, 0.f);
+  for (int axis = 0; axis < 3; ++axis)
+    launch(c, sign < 0 ? "gen_fwd" : "gen_inv", [&] {
+      launch_gen_pass(c->gplan, axis, sign, 0, x, x, c->tw, none, nullptr, nullptr, 0, c->stream);
+    });
+}
+
 int ensure_diff(sf21_hii_ctx* c) {
   if (c->df_ready) return SF21_OK;
   const size_t n3 = (size_t)c->n3, nh = (size_t)c->nh;
@@ -576,11 +586,14 @@ int ensure_diff(sf21_hii_ctx* c) {
       (rc = dev_alloc(&c->df_fa, n3)) || (rc = dev_alloc(&c->df_qbar, n3)) ||
       (rc = dev_alloc(&c->df_dbar, n3)) || (rc = dev_alloc(&c->df_dd, n3)) ||
       (rc = dev_alloc(&c->df_v, npair * n3)) || (rc = dev_alloc(&c->df_c, n3)) ||
-      (rc = dev_alloc(&c->df_g, n3)) || (rc = dev_alloc(&c->df_half, nh)) ||
+      (rc = dev_alloc(&c->df_g, n3)) ||
+      // the P(k) spectrum: the half spectrum, or on generic sides the full one
+      (rc = dev_alloc(&c->df_half, c->generic ? n3 : nh)) ||
       (rc = dev_alloc(&c->df_ball, h * h * h * c->radii.size())) ||
       (rc = dev_alloc(&c->df_st, (size_t)DS_COUNT)) ||
       (rc = dev_alloc(&c->df_part, (size_t)2 * diff_partials_count(c->n3))) ||
-      (rc = dev_alloc(&c->df_rho, (size_t)rows_blocks(c->n))) || (rc = dev_alloc(&c->df_gp, 64)) ||
+      (rc = dev_alloc(&c->df_rho, (size_t)std::max(rows_blocks(c->n), gen_seed_blocks(c->n3)))) ||
+      (rc = dev_alloc(&c->df_gp, 64)) ||
       (rc = dev_alloc(&c->df_coef, 64)) || (rc = dev_alloc(&c->df_fp, 4)))
     return rc;
   if (hipHostMalloc(reinterpret_cast<void**>(&c->df_host), sizeof(double) * 256) != hipSuccess) {
@@ -597,9 +610,13 @@ int ensure_diff(sf21_hii_ctx* c) {
   const int n = c->n;
   for (size_t r = 0; r < c->radii.size(); ++r) {
     launch_ball_box(n, c->d2[r], c->df_c, s);
-    launch_c2c_rows(n, -1, c->df_c, c->tw, s);
-    launch_cols(n, n, 0, -1, c->df_c, c->df_c, 1, 0, c->tw, s, 16, kColsStream);
-    launch_cols(n, n, 1, -1, c->df_c, c->df_c, 1, 0, c->tw, s, 16, kColsStream);
+    if (c->generic) {
+      gen_fft3(c, c->df_c, -1);
+    } else {
+      launch_c2c_rows(n, -1, c->df_c, c->tw, s);
+      launch_cols(n, n, 0, -1, c->df_c, c->df_c, 1, 0, c->tw, s, 16, kColsStream);
+      launch_cols(n, n, 1, -1, c->df_c, c->df_c, 1, 0, c->tw, s, 16, kColsStream);
+    }
     launch_ball_fold(n, c->df_c, c->df_ball + r * h * h * h, s);
   }
   if (hipStreamSynchronize(s) != hipSuccess) return SF21_ERR_HIP;
@@ -736,16 +753,109 @@ void enqueue_diff_backward(sf21_hii_ctx* c, const float* g, const float* emis, d
   });
 }
 
+// The same backward on a generic side: full n^3 complex boxes, 3D transforms
+// by the line passes (gen_fft3), the per-radius steps one radius pair at a
+// time (phi and the ball products share a complex transform per pair), the
+// per-radius fields always recomputed from the forward spectrum Z.
+void enqueue_diff_backward_generic(sf21_hii_ctx* c, const float* g, const float* emis,
+                                   double zeta, double tau, int nbins, double gx, float* grad_g) {
+  hipStream_t s = c->stream;
+  const int n = c->n;
+  const long n3 = c->n3;
+  const int nr = (int)c->radii.size();
+  const long h = n / 2 + 1, h3 = h * h * h;
+  const float inv_tau = (float)(1.0 / tau);
+  const CellConsts k = cell_consts(c, 0.0);  // direct inputs
+  const GenPlan& gp = c->gplan;
+  const float2 none = make_float2(0.f, 0.f);
+  enqueue_diff_fields(c, g, emis, zeta);
+  // forward spectrum Z of rec + i s ion
+  launch(c, "balance", [&] { launch_balance(n3, c->df_rec, c->df_ion, nullptr, k, c->bal_part, c->bal, s); });
+  launch(c, "gen_pack", [&] { launch_gen_pack(n3, c->df_rec, c->df_ion, nullptr, k, c->bal, c->spec, s); });
+  gen_fft3(c, c->spec, -1);
+  // the real-space pair (N R_R, s N I_R) of radius r into ubuf
+  auto pair_of = [&](int r) {
+    launch(c, "gen_filter_i", [&] {
+      launch_gen_pass(gp, 2, +1, 1, c->spec, c->ubuf, c->tw, c->kap[r], nullptr, nullptr, 0, s);
+    });
+    launch(c, "gen_inv_j", [&] { launch_gen_pass(gp, 1, +1, 0, c->ubuf, c->ubuf, c->tw, none, nullptr, nullptr, 0, s); });
+    launch(c, "gen_inv_p", [&] { launch_gen_pass(gp, 0, +1, 0, c->ubuf, c->ubuf, c->tw, none, nullptr, nullptr, 0, s); });
+  };
+  // sweep 1: C = sum_R B_R FFT(phi_R)
+  launch(c, "diff_fill", [&] { launch_fill_c(c->df_c, n3, s); });
+  for (int r = 0; r < nr; r += 2) {
+    const bool two = r + 1 < nr;
+    for (int hh = 0; hh < (two ? 2 : 1); ++hh) {
+      pair_of(r + hh);
+      launch(c, "gen_phi", [&] { launch_gen_phi(n3, c->ubuf, c->df_v, hh, c->bal, c->df_st, inv_tau, s); });
+    }
+    gen_fft3(c, c->df_v, -1);
+    launch(c, "gen_ball_acc", [&] {
+      launch_gen_ball(n, true, c->df_v, c->df_c, c->df_ball + (long)r * h3,
+                      two ? c->df_ball + (long)(r + 1) * h3 : nullptr, s);
+    });
+  }
+  gen_fft3(c, c->df_c, +1);
+  // relaxed state, its P(k) and the observables' adjoints
+  launch(c, "relaxed_state", [&] {
+    launch_relaxed_state(n3, c->df_c, c->df_rec, c->df_ion, c->df_delta, c->df_x, c->df_t,
+                         c->df_fq, c->df_fa, c->df_part, c->df_st, s);
+  });
+  enqueue_pk(c, c->df_t, c->df_half, nbins);
+  const double dl = c->dx;
+  const double norm = dl * dl * dl / ((double)n3);
+  launch(c, "pk_adjoint", [&] {
+    launch_pk_coef(c->df_gp, c->pk_out, nbins, norm, c->df_coef, s);
+    launch_pk_adjoint_full(n, c->df_half, c->pk_bin, c->df_coef, s);
+  });
+  gen_fft3(c, c->df_half, +1);
+  launch(c, "real_part", [&] { launch_real_part(n3, c->df_half, c->df_t, s); });  // dL/dT
+  launch(c, "seed_a", [&] {
+    launch_seed_a(n3, c->df_t, c->df_delta, c->df_x, c->df_fq, c->df_fa, (float)(gx / (double)n3),
+                  c->df_c, c->df_qbar, c->df_dbar, s);
+  });
+  gen_fft3(c, c->df_c, -1);
+  // sweep 2: G = sum_R W_R FFT(seed_R)
+  const int nrho = gen_seed_blocks(n3);
+  launch(c, "diff_fill", [&] {
+    launch_fill_c(c->df_g, n3, s);
+    launch_fill_d(c->df_rho, nrho, s);
+  });
+  for (int r = 0; r < nr; r += 2) {
+    const bool two = r + 1 < nr;
+    launch(c, "gen_ball", [&] {
+      launch_gen_ball(n, false, c->df_c, c->df_v, c->df_ball + (long)r * h3,
+                      two ? c->df_ball + (long)(r + 1) * h3 : nullptr, s);
+    });
+    gen_fft3(c, c->df_v, +1);  // N (phibar_a + i phibar_b)
+    for (int hh = 0; hh < (two ? 2 : 1); ++hh) {
+      pair_of(r + hh);
+      launch(c, "gen_seed", [&] {
+        launch_gen_seed(n3, c->ubuf, c->df_v, hh, c->bal, c->df_st, inv_tau, c->df_rho, s);
+      });
+      gen_fft3(c, c->ubuf, -1);
+      launch(c, "gen_filter_acc", [&] { launch_gen_filter_acc(n, c->ubuf, c->df_g, c->kap[r + hh], s); });
+    }
+  }
+  gen_fft3(c, c->df_g, +1);
+  launch(c, "grad_cells", [&] {
+    launch_sum_rho(c->df_rho, nrho, c->df_st, s);
+    launch_grad_cells(n3, c->df_g, c->df_qbar, c->df_rec, c->df_ion, emis, c->df_delta, c->df_dbar,
+                      c->df_st, c->df_dd, c->df_part, s);
+    launch_grad_g(n3, c->df_delta, c->df_dd, c->df_st, grad_g, s);
+  });
+}
+
 int diff_check(sf21_hii_ctx* c, const float* g, const float* emis, int nbins) {
   if (!c || !g || !emis || nbins < 1 || nbins > 64) return SF21_ERR_ARG;
   if (c->cfg.input_mode != SF21_INPUT_DIRECT || c->cfg.path != SF21_PATH_FUSED) return SF21_ERR_ARG;
-  if (c->n < 64 || !diff_size_supported(c->n)) return SF21_ERR_ARG;
+  if (c->n < 64 || !(c->generic || diff_size_supported(c->n))) return SF21_ERR_ARG;
   if (hipSetDevice(c->cfg.device) != hipSuccess) return SF21_ERR_HIP;
   (void)hipGetLastError();
   int rc;
   if ((rc = ensure_pk(c, nbins))) return rc;
   if ((rc = ensure_diff(c))) return rc;
-  if (c->df_store == 2 && !c->df_pairs) {
+  if (c->df_store == 2 && !c->df_pairs && !c->generic) {
     // automatic field storage: keep the pairs when they take at most half
     // of the free device memory
     size_t free_b = 0, total_b = 0;
@@ -821,6 +931,35 @@ void tune_store(const char* path, const std::string& key, const TuneChoice& t, d
   if (!f) return;
   std::fprintf(f, "%s | tile=%d K=%d ct=%d cw=%d rw=%d fp=%d | %.3f ms\n", key.c_str(),
                t.tile, t.K, t.ct, t.cw, t.rw, t.fp, ms);
+  std::fclose(f);
+}
+
+// Sides without a register plan: "<key> | gx=M | <ms> ms", M the XCD-order
+// mask of the line passes (GenPlan::xcd); the last match wins.
+bool tune_lookup_gen(const char* path, const std::string& key, int* gx) {
+  FILE* f = path ? std::fopen(path, "r") : nullptr;
+  if (!f) return false;
+  char line[512];
+  bool found = false;
+  while (std::fgets(line, sizeof(line), f)) {
+    const char* bar = std::strchr(line, '|');
+    if (!bar) continue;
+    std::string k(line, (size_t)(bar - line));
+    while (!k.empty() && k.back() == ' ') k.pop_back();
+    int m = 0;
+    if (k == key && std::sscanf(bar + 1, " gx=%d", &m) == 1 && m >= 0 && m <= 6 && (m & 1) == 0) {
+      *gx = m;
+      found = true;
+    }
+  }
+  std::fclose(f);
+  return found;
+}
+
+void tune_store_gen(const char* path, const std::string& key, int gx, double ms) {
+  FILE* f = path ? std::fopen(path, "a") : nullptr;
+  if (!f) return;
+  std::fprintf(f, "%s | gx=%d | %.3f ms\n", key.c_str(), gx, ms);
   std::fclose(f);
 }
 
@@ -915,9 +1054,60 @@ int create_deposit(const sf21_hii_config* cfg, sf21_hii_ctx** out) {
 
 // Entry points that need the transforms and the bubble finder's buffers.
 bool is_deposit(const sf21_hii_ctx* c) { return c && c->cfg.path == SF21_PATH_DEPOSIT; }
-// Entry points built on the register transform plans (autotuner,
-// differentiable mode).
+// Sides without a register plan: the fused finder runs one radius per step
+// through the line passes; the autotuner searches their block order, and the
+// differentiable mode recomputes its fields there.
 bool is_generic(const sf21_hii_ctx* c) { return c && c->generic; }
+
+// Autotuner on a side without a register plan: the XCD-order mask of the
+// line passes (block order only: results bit-identical), the current mask
+// first, then the others; one warm-up and two timed slices each.
+int autotune_generic(sf21_hii_ctx* c, const float* a, const float* b, const float* cc, double z,
+                     const char* cache_path, double* best_ms) {
+  float* xout = nullptr;
+  int rc = dev_alloc(&xout, (size_t)c->n3);
+  if (rc) return rc;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    if (e0) (void)hipEventDestroy(e0);
+    dev_free(xout);
+    return SF21_ERR_HIP;
+  }
+  const bool prof = c->prof;
+  c->prof = false;
+  std::vector<int> masks{c->gplan.xcd};
+  for (int m : {0, 2, 4, 6})
+    if (m != c->gplan.xcd) masks.push_back(m);
+  int best = c->gplan.xcd;
+  double best_t = 1e300;
+  for (int m : masks) {
+    c->gplan.xcd = m;
+    drop_graphs(c);
+    enqueue_fused(c, a, b, cc, z, xout, c->neutral_dev, -1, nullptr);
+    (void)hipEventRecord(e0, c->stream);
+    for (int r = 0; r < 2; ++r) enqueue_fused(c, a, b, cc, z, xout, c->neutral_dev, -1, nullptr);
+    (void)hipEventRecord(e1, c->stream);
+    float ms = 0.f;
+    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+      rc = SF21_ERR_HIP;
+      break;
+    }
+    if (ms / 2 < best_t) {
+      best_t = ms / 2;
+      best = m;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  dev_free(xout);
+  c->prof = prof;
+  c->gplan.xcd = best;
+  drop_graphs(c);
+  if (rc) return rc;
+  tune_store_gen(tune_path(cache_path), tune_key(c), best, best_t);
+  if (best_ms) *best_ms = best_t;
+  return check_hip();
+}
 
 }  // namespace
 
@@ -982,6 +1172,10 @@ int sf21_hii_create(const sf21_hii_config* cfg, sf21_hii_ctx** out) {
       cfg->filter_tile == 0 && cfg->filter_w == 0)
     tuned = tune_lookup(tune_path(nullptr), tune_key(c), &tc);
   if (tuned) c->K = tc.K;
+  if (c->generic && cfg->filter_tile == 0 && cfg->filter_w == 0) {
+    int gx = 0;
+    if (tune_lookup_gen(tune_path(nullptr), tune_key(c), &gx)) c->gplan.xcd = gx;
+  }
   if (cfg->filter_tile == 4 || cfg->filter_tile == 8 || cfg->filter_tile == 16)
     c->tile_filter = cfg->filter_tile;
   c->filter_w = cfg->filter_w == 2 ? 0 : 1;
@@ -1375,10 +1569,11 @@ int sf21_hii_filtered_dev(sf21_hii_ctx* c, const float* a, const float* b, const
 
 int sf21_hii_autotune(sf21_hii_ctx* c, const float* a, const float* b, const float* cc, double z,
                       const char* cache_path, double* best_ms) {
-  if (is_deposit(c) || is_generic(c)) return SF21_ERR_ARG;
+  if (is_deposit(c)) return SF21_ERR_ARG;
   if (!c || !a || !b || c->cfg.path != SF21_PATH_FUSED) return SF21_ERR_ARG;
   if (hipSetDevice(c->cfg.device) != hipSuccess) return SF21_ERR_HIP;
   (void)hipGetLastError();
+  if (is_generic(c)) return autotune_generic(c, a, b, cc, z, cache_path, best_ms);
   float* xout = nullptr;
   int rc = dev_alloc(&xout, (size_t)c->n3);
   if (rc) return rc;
@@ -1485,6 +1680,10 @@ int sf21_hii_tuning(const sf21_hii_ctx* c, int* filter_tile, int* radius_batch, 
 }
 
 int sf21_hii_radius_batch(const sf21_hii_ctx* c) { return c ? c->K : SF21_ERR_ARG; }
+
+int sf21_hii_line_order(const sf21_hii_ctx* c) {
+  return is_generic(c) ? c->gplan.xcd : SF21_ERR_ARG;
+}
 
 int sf21_hii_profile(sf21_hii_ctx* c, int enable) {
   if (!c) return SF21_ERR_ARG;
@@ -1889,8 +2088,10 @@ int sf21_fft_destroy(sf21_fft_plan* p) {
 
 // ---- differentiable mode ---------------------------------------------------
 int sf21_hii_diff_store_fields(sf21_hii_ctx* c, int mode) {
-  if (is_deposit(c) || is_generic(c)) return SF21_ERR_ARG;
+  if (is_deposit(c)) return SF21_ERR_ARG;
   if (!c || mode < 0 || mode > 3) return SF21_ERR_ARG;
+  // generic sides always recompute: automatic (2) stores nothing there
+  if (is_generic(c) && mode != 0 && mode != 2) return SF21_ERR_ARG;
   if (hipSetDevice(c->cfg.device) != hipSuccess) return SF21_ERR_HIP;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   c->df_store = mode;
@@ -1910,7 +2111,7 @@ int sf21_hii_diff_last_kept(const sf21_hii_ctx* c) {
 
 int sf21_hii_diff_forward_dev(sf21_hii_ctx* c, const float* g, const float* emis, double zeta,
                               int nbins, float* x_out, double* mean_x, double* pk) {
-  if (is_deposit(c) || is_generic(c)) return SF21_ERR_ARG;
+  if (is_deposit(c)) return SF21_ERR_ARG;
   int rc = diff_check(c, g, emis, nbins);
   if (rc) return rc;
   float* x = x_out ? x_out : c->df_x;
@@ -1942,7 +2143,7 @@ int sf21_hii_diff_backward_dev(sf21_hii_ctx* c, const float* g, const float* emi
                                double tau, int nbins, double dl_dx, const double* dl_dpk,
                                float* grad_g, double* grad_zeta, double* relaxed_mean_x,
                                double* relaxed_pk) {
-  if (is_deposit(c) || is_generic(c)) return SF21_ERR_ARG;
+  if (is_deposit(c)) return SF21_ERR_ARG;
   int rc = diff_check(c, g, emis, nbins);
   if (rc) return rc;
   if (!grad_g || !(tau > 0.0)) return SF21_ERR_ARG;
@@ -1967,7 +2168,10 @@ int sf21_hii_diff_backward_dev(sf21_hii_ctx* c, const float* g, const float* emi
     c->df_kept_zeta = zeta;
   }
   c->df_kept = false;  // set again below once the pairs are this input's
-  enqueue_diff_backward(c, g, emis, zeta, tau, nbins, dl_dx, grad_g, kept);
+  if (c->generic)
+    enqueue_diff_backward_generic(c, g, emis, zeta, tau, nbins, dl_dx, grad_g);
+  else
+    enqueue_diff_backward(c, g, emis, zeta, tau, nbins, dl_dx, grad_g, kept);
   // the staging buffer is reused for the results only after the stream has
   // consumed the upload above (same stream, ordered)
   if (hipMemcpyAsync(c->df_host + 64, c->df_st, sizeof(double) * DS_COUNT, hipMemcpyDeviceToHost,

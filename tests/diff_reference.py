@@ -56,13 +56,20 @@ def w_filter(x):
     return 3.0 * np.where(x < 0.03, 1.0 / 3.0 - x * x / 30.0, big)
 
 
+def freqs(n):
+    """Integer DFT frequencies in numpy's order (fftfreq(n) * n rounds off
+    the integers at sides that are not powers of two, which moves cells with
+    |d|^2 exactly d2max out of the ball)."""
+    return np.rint(np.fft.fftfreq(n) * n).astype(np.int64)
+
+
 def k2_full(n):
-    f = np.fft.fftfreq(n) * n
+    f = freqs(n)
     return f[:, None, None] ** 2 + f[None, :, None] ** 2 + f[None, None, :] ** 2
 
 
 def ball_hat(n, d2):
-    f = np.fft.fftfreq(n) * n
+    f = freqs(n)
     a = np.minimum(np.abs(f), n - np.abs(f))
     m = a[:, None, None] ** 2 + a[None, :, None] ** 2 + a[None, None, :] ** 2
     ball = (m <= d2).astype(np.float64)
@@ -70,7 +77,7 @@ def ball_hat(n, d2):
 
 
 def pk_bins(n, nbins):
-    kx = np.fft.fftfreq(n) * n
+    kx = freqs(n)
     kq = np.sqrt(kx[:, None, None] ** 2 + kx[None, :, None] ** 2 + np.arange(n // 2 + 1)[None, None, :] ** 2)
     b = np.full(kq.shape, -1, np.int64)
     sel = (kq >= 1) & (kq <= n // 2)

@@ -13,7 +13,9 @@ sides.
     wave through the filtered-field hook equals W(|k| R) times the wave
     (closed form), and the xHII box of a synthetic slice equals the exact
     dilation of the GPU's own centre map merged with the quick fill, bit for
-    bit, with the neutral fraction its FP64 mean."""
+    bit, with the neutral fraction its FP64 mean;
+  * the autotuner's choice on these sides (the line passes' XCD block
+    order): cached, reloaded, and bit-identical for every mask."""
 import numpy as np
 import pytest
 
@@ -104,22 +106,59 @@ def test_generic_600_dilation_and_mean(torch_cuda):
 
 
 @pytest.mark.gpu
-def test_generic_refusals(torch_cuda):
-    """Entry points built on the register plans (autotuner, differentiable
-    mode) refuse generic sides with an argument error instead of running."""
+def test_generic_autotune_records_and_reloads(torch_cuda, tmp_path):
+    """On a side without a register plan the autotuner times the four
+    XCD-order masks of the line passes (block order only): the winner is
+    recorded in the cache file, a new context created with automatic
+    settings picks it up, and every mask gives the untuned box bit for bit."""
+    import os
     torch = torch_cuda
     n = 80
     case = direct_case(n, 1.0)
-    ctx = sf.HiiContext(n, case["side"], case["radii"], sf.INPUT_DIRECT, sf.PATH_FUSED)
+    side, radii = case["side"], case["radii"]
     a = torch.from_numpy(case["rec"].ravel()).cuda()
     b = torch.from_numpy(case["ion"].ravel()).cuda()
-    with pytest.raises(sf.Sf21Error):
-        ctx.autotune(a, b)
-    with pytest.raises(sf.Sf21Error):
-        ctx.diff_forward(a, b, 1.0)
-    x, neu = ctx.run_slice(a, b)  # the context still runs
-    assert 0.0 <= neu <= 1.0
+    ctx = sf.HiiContext(n, side, radii, sf.INPUT_DIRECT, sf.PATH_FUSED)
+    assert ctx.line_order == 4
+    x0, neu0 = ctx.run_slice(a, b)
+    x0 = x0.cpu().numpy().copy()
+    cache = tmp_path / "tune.txt"
+    ms = ctx.autotune(a, b, cache_path=cache)
+    m = ctx.line_order
+    print(f"\ngeneric autotune {n}^3: line order {m}, {ms:.3f} ms/slice", flush=True)
+    assert ms > 0 and m in (0, 2, 4, 6)
+    lines = cache.read_text().strip().splitlines()
+    assert len(lines) == 1 and f"n={n} L={side:.17g} radii={len(radii)} ladder=" in lines[0]
+    assert f"| gx={m} |" in lines[0]
+    x1, neu1 = ctx.run_slice(a, b)
+    assert np.array_equal(x1.cpu().numpy(), x0) and neu1 == neu0
+    key = lines[0].split("|")[0].strip()
     ctx.close()
+    old = os.environ.get("SF21_TUNE_CACHE")
+    os.environ["SF21_TUNE_CACHE"] = str(cache)
+    try:
+        ctx2 = sf.HiiContext(n, side, radii, sf.INPUT_DIRECT, sf.PATH_FUSED)
+        assert ctx2.line_order == m
+        ctx2.close()
+        # every mask through the cache (the last matching line wins)
+        for mask in (0, 2, 4, 6):
+            with open(cache, "a") as f:
+                f.write(f"{key} | gx={mask} | 1.000 ms\n")
+            c3 = sf.HiiContext(n, side, radii, sf.INPUT_DIRECT, sf.PATH_FUSED)
+            assert c3.line_order == mask
+            x3, neu3 = c3.run_slice(a, b)
+            assert np.array_equal(x3.cpu().numpy(), x0) and neu3 == neu0, mask
+            c3.close()
+    finally:
+        if old is None:
+            del os.environ["SF21_TUNE_CACHE"]
+        else:
+            os.environ["SF21_TUNE_CACHE"] = old
+    # plan sides have no line order
+    ctx4 = sf.HiiContext(64, side, radii, sf.INPUT_DIRECT, sf.PATH_FUSED)
+    with pytest.raises(sf.Sf21Error):
+        ctx4.line_order
+    ctx4.close()
 
 
 @pytest.mark.gpu

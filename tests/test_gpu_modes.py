@@ -310,7 +310,7 @@ def test_power_spectrum_vs_oracle(torch_cuda, n):
     assert np.allclose(pk, rp, rtol=2e-5), np.max(np.abs(pk / rp - 1))
     # numpy restatement of the same definition
     F = np.fft.rfftn(g)
-    kx = np.fft.fftfreq(n) * n
+    kx = np.rint(np.fft.fftfreq(n) * n)  # exact integers at every side
     kq = np.sqrt(kx[:, None, None] ** 2 + kx[None, :, None] ** 2 + np.arange(n // 2 + 1)[None, None, :] ** 2)
     sel = (kq >= 1) & (kq <= n // 2)
     b = np.minimum(np.floor((kq[sel] - 1) / ((n // 2 - 1) / 20)).astype(int), 19)
