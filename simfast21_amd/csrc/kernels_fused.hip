@@ -401,8 +401,9 @@ __global__ void k_fill_i32(int* __restrict__ p, int value, long count) {
 // bracket of their level-1 neighbours.  A
 // neighbour xa that found no non-negative value bounds the search too: a
 // y < xa with f(y) >= (x-y)^2 > (xa-y)^2 would contradict it.  Only values
-// >= 0 are ever needed, so candidates farther than floor(sqrt(fmax)) are
-// never visited.
+// >= 0 are ever needed, so levels 0 and 1 never look farther than
+// floor(sqrt(fmax)); level 2 scans its bracket once for all of a lane's
+// cells.
 // ------------------------------------------------------------------------
 struct Found {
   int best;  // -1: nothing >= 0
@@ -507,17 +508,28 @@ __device__ __forceinline__ void dt_line_wave(const int* col, int n, int Q, int h
     const int hi = b_best >= 0 ? b_hi0 + (gb < P1 ? 0 : n) : xb;
     l1 = scan_range(x1, lo, hi, hl, f);
   }
-  // level 2: between x1 and the next lane's level-1 point
+  // level 2: the cells x1 + q, 0 < q < Q, all inside the bracket [lo, hi]
+  // between x1's and the next lane's maximisers, in one pass over it.  No
+  // window clip: with h = min(floor(sqrt(max w)), n/2) a candidate outside
+  // x's window is either farther than sqrt(f(y)) (negative value) or an
+  // unrolled image farther than n/2 whose value is below that of its own
+  // cell at the distance n - d, so it never raises a non-negative maximum.
   const int nl = lane + 1 < P1 ? lane + 1 : 0;
   const int n_best = __shfl(l1.best, nl, 64), n_hi0 = __shfl(l1.hi, nl, 64);
   out[0] = l1.best;
 #pragma unroll
-  for (int q = 1; q < QMAX; ++q) {
-    out[q] = -1;
-    if (act && q < Q) {
-      const int lo = l1.best >= 0 ? l1.lo : x1;
-      const int hi = n_best >= 0 ? n_hi0 + (lane + 1 < P1 ? 0 : n) : x1 + Q;
-      out[q] = scan_range(x1 + q, lo, hi, hl, f).best;
+  for (int q = 1; q < QMAX; ++q) out[q] = -1;
+  if (act && Q > 1) {
+    const int lo = l1.best >= 0 ? l1.lo : x1;
+    const int hi = n_best >= 0 ? n_hi0 + (lane + 1 < P1 ? 0 : n) : x1 + Q;
+    for (int y = lo; y <= hi; ++y) {
+      int v = f(y);
+      v = v < 0 ? -(1 << 28) : v;
+#pragma unroll
+      for (int q = 1; q < QMAX; ++q) {
+        const int d = x1 + q - y;
+        out[q] = max(out[q], v - d * d);
+      }
     }
   }
 }
@@ -641,41 +653,40 @@ __global__ __launch_bounds__(256) void k_dt_rows_final(DtArgs a, const int* in,
   int* col = sm + wave * cs;
   double acc = 0.0;
   const long rows = (long)n * n;
-  // block-uniform trip count: every wave reaches both barriers each time
-  for (long r0 = (long)blockIdx.x * nw; r0 < rows; r0 += (long)gridDim.x * nw) {
-    const long row = r0 + wave;
-    const bool valid = row < rows;
-    if (valid) {
-      const int* src = in + row * n;
-      for (int e = lane; e < n; e += 64) col[e + (e >> 3)] = src[e];
-    }
+  // each wave walks its own rows through its own LDS line: only its lanes
+  // exchange data there, and the LDS operations of one wave complete in
+  // issue order, so compiler fences suffice and the waves run independently
+  const long stride = (long)gridDim.x * nw;
+  for (long row = (long)blockIdx.x * nw + wave; row < rows; row += stride) {
+    const int* src = in + row * n;
+    for (int e = lane; e < n; e += 64) col[e + (e >> 3)] = src[e];
     lds_pin();
-    __syncthreads();
+    __builtin_amdgcn_wave_barrier();
     int o[QMAX];
-    if (valid) cover_line_wave<QMAX>(col, n, Q, a.h, lane, o);
-    if (valid) {
-      // the lane's Q consecutive flags go back into the wave's own column
-      // (the line's reads of it are complete), then lane takes cells
-      // lane, lane + 64, ...: coalesced loads of the fields and stores of x
-      if (lane < n / Q)
-        for (int q = 0; q < Q; ++q) {
-          const int e = Q * lane + q;
-          col[e + (e >> 3)] = o[q];
-        }
-      lds_pin();
-      __builtin_amdgcn_wave_barrier();
-      for (int e = lane; e < n; e += 64) {
-        const long idx = row * n + e;
-        float rec, ion, fres;
-        cell_fields(k, pa, pb, pc, idx, rec, ion, fres);
-        float v = col[e + (e >> 3)] ? 1.0f : quick_fill(rec, ion);
-        v = fminf(v, 1.0f - fres);
-        xhii[idx] = v;
-        acc += 1.0 - (double)v;
+    cover_line_wave<QMAX>(col, n, Q, a.h, lane, o);
+    lds_pin();
+    __builtin_amdgcn_wave_barrier();
+    // the lane's Q consecutive flags go back into the wave's own line (its
+    // reads are complete), then lane takes cells lane, lane + 64, ...:
+    // coalesced loads of the fields and stores of x
+    if (lane < n / Q)
+      for (int q = 0; q < Q; ++q) {
+        const int e = Q * lane + q;
+        col[e + (e >> 3)] = o[q];
       }
+    lds_pin();
+    __builtin_amdgcn_wave_barrier();
+    for (int e = lane; e < n; e += 64) {
+      const long idx = row * n + e;
+      float rec, ion, fres;
+      cell_fields(k, pa, pb, pc, idx, rec, ion, fres);
+      float v = col[e + (e >> 3)] ? 1.0f : quick_fill(rec, ion);
+      v = fminf(v, 1.0f - fres);
+      xhii[idx] = v;
+      acc += 1.0 - (double)v;
     }
     lds_pin();
-    __syncthreads();
+    __builtin_amdgcn_wave_barrier();
   }
   __shared__ double red[256];
   red[threadIdx.x] = acc;
