@@ -79,3 +79,28 @@ def test_relaxed_model_tends_to_exact_forward():
     # away from near-threshold centres the relaxed value equals the exact one
     assert np.mean(diff < 1e-3) > 0.995
     assert abs(float(X) - (1 - neu)) < 5e-3
+
+
+@pytest.mark.parametrize("n", [24, 40, 80])
+def test_ball_and_bins_at_non_power_of_two_sides(n):
+    """At sides that are not powers of two (fftfreq(n) * n is not exactly
+    integral there): ball_hat's zero mode is the lattice-point count
+    #{d in Z^3 : |d|^2 <= d2} and its value at a few wave vectors is the
+    explicit sum of cos(2 pi k.d / n) over those points (d2 on lattice shells,
+    below (n/2)^2); pk_bins keeps exactly the half-spectrum modes with
+    1 <= |k|^2 <= (n/2)^2, counted with integer arithmetic."""
+    h = n // 2
+    for d2 in (1, 2, 3, 25, 50, (n // 3) ** 2):
+        r = int(np.floor(np.sqrt(d2)))
+        pts = [(a, b, c) for a in range(-r, r + 1) for b in range(-r, r + 1)
+               for c in range(-r, r + 1) if a * a + b * b + c * c <= d2]
+        B = dr.ball_hat(n, d2)
+        assert abs(B[0, 0, 0] - len(pts)) < 1e-9, (n, d2)
+        P = np.array(pts, np.float64)
+        for k in ((1, 0, 0), (0, 3, 5), (h, 1, n - 2), (7, h, h)):
+            ref = np.cos(2 * np.pi * (P @ np.array(k, np.float64)) / n).sum()
+            assert abs(B[k] - ref) < 1e-8 * max(1.0, len(pts)), (n, d2, k)
+    b = dr.pk_bins(n, 20)
+    m = sum(1 for i in range(n) for j in range(n) for p in range(h + 1)
+            if 1 <= min(i, n - i) ** 2 + min(j, n - j) ** 2 + p * p <= h * h)
+    assert int((b >= 0).sum()) == m
