@@ -340,7 +340,7 @@ def bench_diff(torch, sf, n, side, radii, n_src, taus=(0.05, 0.02, 0.01), reps=3
     }
 
 
-def bench_generic(torch, sf, synth, n=600, steps=2, seed=6006):
+def bench_generic(torch, sf, synth, n=600, steps=2, seed=6006, tune=True):
     """One slice at a box side without a register plan (600 = 4*2*3*5*5, the
     N_smoothed of SimFast21's sample ini): the same recipe scaled to n, one
     radius per step through LDS line transforms."""
@@ -352,8 +352,9 @@ def bench_generic(torch, sf, synth, n=600, steps=2, seed=6006):
     torch.cuda.empty_cache()
     ctx = sf.HiiContext(n, side, radii, sf.INPUT_DIRECT, sf.PATH_FUSED)
     out = torch.empty_like(recs[0])
-    # the line passes' XCD block order (results identical for every choice)
-    tune_ms = ctx.autotune(recs[0], ions[0])
+    # the line passes' XCD block order (results identical for every choice);
+    # without tuning, the choice cached in $SF21_TUNE_CACHE (or the default)
+    tune_ms = ctx.autotune(recs[0], ions[0]) if tune else None
     order = ctx.line_order
     ctx.run_slice(recs[0], ions[0], xhii=out)
     torch.cuda.synchronize()
@@ -379,7 +380,7 @@ def bench_generic(torch, sf, synth, n=600, steps=2, seed=6006):
         kern[k] = e
     return {"n": n, "side": side, "radii": len(radii), "ms_per_slice": round(ms, 2),
             "voxel_radius_per_s": v * len(radii) / (ms * 1e-3), "mean_xhii": round(1 - neu, 6),
-            "line_order": order, "tune_ms_per_slice": round(tune_ms, 2),
+            "line_order": order, "tune_ms_per_slice": None if tune_ms is None else round(tune_ms, 2),
             "algorithmic_bytes_per_voxel_radius": 40, "kernels": kern}
 
 
@@ -627,7 +628,7 @@ def main():
     generic = None
     if not args.no_generic:
         try:
-            generic = bench_generic(torch, sf, synth)
+            generic = bench_generic(torch, sf, synth, tune=not args.no_tune)
         finally:
             torch.cuda.empty_cache()
         print(f"[bench] generic side {generic['n']}: {generic['ms_per_slice']:.1f} ms/slice, "
