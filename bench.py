@@ -342,8 +342,11 @@ def bench_diff(torch, sf, n, side, radii, n_src, taus=(0.05, 0.02, 0.01), reps=3
 
 def bench_generic(torch, sf, synth, n=600, steps=2, seed=6006, tune=True):
     """One slice at a box side without a register plan (600 = 4*2*3*5*5, the
-    N_smoothed of SimFast21's sample ini): the same recipe scaled to n, one
-    radius per step through LDS line transforms."""
+    N_smoothed of SimFast21's sample ini): the same recipe scaled to n (its
+    emissivity scaled to mean x_HII ~0.5, untimed), one radius per step
+    through LDS line transforms; then the differentiable mode on the same
+    fields (g = log(rec)/2, the scaled emissivity, zeta 1): one exact forward
+    and one relaxed backward (tau 0.02, fields recomputed) after a warm-up."""
     side = 400.0 * n / 512
     radii = synth.geometric_ladder(100.0 * n / 512, 0.9, side / n)
     n_src = int(round(2.5e5 * (n / 512) ** 3))
@@ -352,6 +355,7 @@ def bench_generic(torch, sf, synth, n=600, steps=2, seed=6006, tune=True):
     torch.cuda.empty_cache()
     ctx = sf.HiiContext(n, side, radii, sf.INPUT_DIRECT, sf.PATH_FUSED)
     out = torch.empty_like(recs[0])
+    zeta = scale_slices(torch, ctx, recs, ions)[0]
     # the line passes' XCD block order (results identical for every choice);
     # without tuning, the choice cached in $SF21_TUNE_CACHE (or the default)
     tune_ms = ctx.autotune(recs[0], ions[0]) if tune else None
@@ -369,6 +373,24 @@ def bench_generic(torch, sf, synth, n=600, steps=2, seed=6006, tune=True):
     torch.cuda.synchronize()
     stats = ctx.kernel_stats()
     ctx.profile(False)
+    # differentiable mode on the same fields (Delta = exp(g)/<exp(g)> = the
+    # slice's (1 + delta), ion = the scaled emissivity)
+    g = (0.5 * torch.log(recs[0])).contiguous()
+    grad = torch.empty_like(g)
+    ctx.diff_forward(g, ions[0], 1.0, 20, x_out=out)
+    ctx.diff_backward(g, ions[0], 1.0, 0.02, 1.0, None, 20, grad_out=grad)  # ball tables
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, dmx, _ = ctx.diff_forward(g, ions[0], 1.0, 20, x_out=out)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ctx.diff_backward(g, ions[0], 1.0, 0.02, 1.0, None, 20, grad_out=grad)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    diff = {"forward_ms": round((t1 - t0) * 1e3, 2), "backward_ms": round((t2 - t1) * 1e3, 2),
+            "grad_over_forward": round((t2 - t1) / (t1 - t0), 3), "mean_x": round(dmx, 5),
+            "tau": 0.02, "fields": "recomputed"}
+    del g, grad
     ctx.close()
     v = n ** 3
     algo = {"gen_filter_i": 16 * v, "gen_inv_j": 16 * v, "gen_inv_p_crit": 8 * v, "gen_fwd": 16 * v}
@@ -381,6 +403,7 @@ def bench_generic(torch, sf, synth, n=600, steps=2, seed=6006, tune=True):
     return {"n": n, "side": side, "radii": len(radii), "ms_per_slice": round(ms, 2),
             "voxel_radius_per_s": v * len(radii) / (ms * 1e-3), "mean_xhii": round(1 - neu, 6),
             "line_order": order, "tune_ms_per_slice": None if tune_ms is None else round(tune_ms, 2),
+            "zeta": round(zeta, 4), "diff": diff,
             "algorithmic_bytes_per_voxel_radius": 40, "kernels": kern}
 
 
