@@ -243,6 +243,16 @@ int sf21_hii_diff_backward_dev(sf21_hii_ctx *ctx, const float *g, const float *e
  * largest radius at which the cell was a bubble centre in the last slice, -1
  * if never (host output, n^3 ints). */
 int sf21_hii_weight_map(sf21_hii_ctx *ctx, int *host_out);
+/* Inspection for tests (fused path): the exact periodic dilation and the
+ * final per-cell pass of a slice (quick fill, clamp, FP64 mean) on a given
+ * map instead of the finder's.  host_map (n^3 ints): per cell -1 or a ball
+ * bound d2 -- every x with min-image |x - y|^2 <= d2 of such a cell y is
+ * ionised -- each at most the ladder's largest d2max (SF21_ERR_ARG
+ * otherwise); a, b, c, z as for sf21_hii_run_slice_dev, xhii (device) and
+ * *neutral as there.  No counterpart in the reference, which dilates per
+ * radius (get_HIIbubbles.c:358-393, :428-458). */
+int sf21_hii_dilate_dev(sf21_hii_ctx *ctx, const int *host_map, const float *a, const float *b,
+                        const float *c, double z, float *xhii, double *neutral);
 /* Number of hipGraphs captured so far with ctx as their owner (batches on
  * ctx, multi-box submissions whose first context is ctx).  A cached graph
  * is replayed only while every context it uses keeps its buffers, stream

@@ -118,6 +118,7 @@ def _lib():
     L.sf21_hii_run_slice_dev.argtypes = [vp, vp, vp, vp, d, vp, dp]
     L.sf21_hii_run_batch_dev.argtypes = [vp, i, vp, vp, vp, dp, vp, dp]
     L.sf21_hii_weight_map.argtypes = [vp, vp]
+    L.sf21_hii_dilate_dev.argtypes = [vp, vp, vp, vp, vp, d, vp, dp]
     L.sf21_hii_graph_captures.argtypes = [vp, ctypes.POINTER(ctypes.c_long)]
     L.sf21_hii_run_multi_dev.argtypes = [i, vp, vp, vp, vp, dp, vp, dp]
     L.sf21_hii_filtered_dev.argtypes = [vp, vp, vp, vp, d, i, vp]
@@ -397,6 +398,22 @@ class HiiContext:
                                    _ptr(xalpha), _ptr(tk), _ptr(out), ctypes.byref(mt), ctypes.byref(ms)),
                "sf21_t21_dev")
         return out, mt.value, ms.value
+
+    def dilate(self, wmap: np.ndarray, a, b, c=None, z: float = 0.0, xhii=None):
+        """The dilation and final pass on a given per-cell ball-bound map
+        (host, n^3 ints, -1 = no centre); device inputs as run_slice ->
+        (xhii, neutral)."""
+        import torch
+        w = np.ascontiguousarray(wmap, np.int32).ravel()
+        if w.size != self.n ** 3:
+            raise ValueError("map size")
+        if xhii is None:
+            xhii = torch.empty_like(a)
+        torch.cuda.current_stream().synchronize()
+        neu = ctypes.c_double()
+        _check(_lib().sf21_hii_dilate_dev(self._h, w.ctypes.data, _ptr(a), _ptr(b), _ptr(c), float(z),
+                                          _ptr(xhii), ctypes.byref(neu)), "sf21_hii_dilate_dev")
+        return xhii, neu.value
 
     def weight_map(self) -> np.ndarray:
         out = np.empty(self.n ** 3, np.int32)

@@ -1548,6 +1548,39 @@ int sf21_hii_weight_map(sf21_hii_ctx* c, int* host_out) {
   return SF21_OK;
 }
 
+int sf21_hii_dilate_dev(sf21_hii_ctx* c, const int* host_map, const float* a, const float* b,
+                        const float* cc, double z, float* xhii, double* neutral) {
+  if (is_deposit(c)) return SF21_ERR_ARG;
+  if (!c || !host_map || !a || !b || !xhii || c->cfg.path != SF21_PATH_FUSED) return SF21_ERR_ARG;
+  // the line passes' reach cap h = min(floor(sqrt(largest d2max)), n/2) is
+  // exact only for bounds up to the ladder's largest
+  const int wmax = *std::max_element(c->d2.begin(), c->d2.end());
+  for (long i = 0; i < c->n3; ++i)
+    if (host_map[i] < -1 || host_map[i] > wmax) return SF21_ERR_ARG;
+  if (hipSetDevice(c->cfg.device) != hipSuccess) return SF21_ERR_HIP;
+  (void)hipGetLastError();
+  hipStream_t s = c->stream;
+  const long n3 = c->n3;
+  const CellConsts k = cell_consts(c, z);
+  if (hipMemcpyAsync(c->wmap, host_map, sizeof(int) * n3, hipMemcpyHostToDevice, s) != hipSuccess)
+    return SF21_ERR_HIP;
+  launch(c, "dt_rows", [&] { launch_dt_rows(c->dt, c->wmap, c->gbuf, s); });
+  launch(c, "dt_cols", [&] { launch_dt_cols(c->dt, c->gbuf, c->gbuf, s); });
+  launch(c, "dt_final", [&] {
+    launch_dt_final(c->dt, c->gbuf, a, b, cc, k, xhii, c->partials, c->npart, s);
+  });
+  launch(c, "sum", [&] {
+    launch_sum_partials(c->partials, c->npart, 1.0 / (double)n3, c->neutral_dev, s);
+  });
+  if (hipMemcpyAsync(c->neutral_host, c->neutral_dev, sizeof(double), hipMemcpyDeviceToHost, s) !=
+          hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return SF21_ERR_HIP;
+  if (neutral) *neutral = c->neutral_host[0];
+  if (c->prof) collect(c);
+  return check_hip();
+}
+
 int sf21_hii_graph_captures(const sf21_hii_ctx* c, long* captures) {
   if (!c || !captures) return SF21_ERR_ARG;
   *captures = c->captures;
