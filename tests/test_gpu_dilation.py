@@ -5,9 +5,9 @@ oracle's separable max-plus transform at 600^3), and the neutral fraction
 is its FP64 mean.
 
 The maps are chosen to reach what the finder's maps at the bench's
-ionisation levels do not: bounds up to 3 (n/2)^2, so the line passes run at
-their reach cap n/2 and meet periodic images from both sides (a ball of
-bound 0.6 * 3 (n/2)^2 leaves the far corners of the box uncovered), centres
+ionisation levels do not: bounds up to 1.8 (n/2)^2, so the line passes run
+at their reach cap n/2 and meet periodic images from both sides (a ball of
+bound 1.8 (n/2)^2 leaves the far corners of the box uncovered), centres
 on the box's corners and edges, equal bounds (ties of the maximiser), bound
 0 (a centre covering only itself), dense small bounds, and the empty map.
 Sides: 64, 80, 96, 120, 128 and 600 -- 1, 2, 2, 3, 2 and 15 cells per lane
