@@ -30,6 +30,7 @@ observables: X = <x>,  T = (1 - x) Delta,  P_b = binned power spectrum of T
 The reference evaluates everything with x~ (the relaxed model) so that
 autograd gives the gradient the GPU adjoint computes.
 """
+import copy
 import math
 
 import numpy as np
@@ -136,11 +137,23 @@ def relaxed(setup, g, zeta, tau, nbins=20, exact_x=None):
     return X, P, x
 
 
-def loss_and_grad(setup, g_np, zeta, tau, cx, cp, nbins=20):
+def single(setup):
+    """The same setup with float32 tensors: with float32 inputs, relaxed()
+    and loss_and_grad() evaluate the model in single precision (the error an
+    FP32 evaluation of it carries)."""
+    s = copy.copy(setup)
+    s.W = [w.float() for w in setup.W]
+    s.B = [b.float() for b in setup.B]
+    s.E = setup.E.float()
+    return s
+
+
+def loss_and_grad(setup, g_np, zeta, tau, cx, cp, nbins=20, dtype=torch.float64):
     """L = cx * X + sum_b cp[b] * P_b of the relaxed model, its gradient in
-    g (numpy box) and zeta (float), FP64 autograd."""
-    g = torch.tensor(np.asarray(g_np, np.float64).reshape(setup.n, setup.n, setup.n), requires_grad=True)
-    z = torch.tensor(float(zeta), dtype=torch.float64, requires_grad=True)
+    g (numpy box) and zeta (float), autograd in dtype (FP64 by default)."""
+    g = torch.tensor(np.asarray(g_np, np.float64).reshape(setup.n, setup.n, setup.n), dtype=dtype,
+                     requires_grad=True)
+    z = torch.tensor(float(zeta), dtype=dtype, requires_grad=True)
     X, P, _ = relaxed(setup, g, z, tau, nbins)
     L = cx * X + (torch.as_tensor(np.asarray(cp, np.float64)) * P).sum()
     L.backward()
