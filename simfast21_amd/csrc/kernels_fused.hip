@@ -719,8 +719,9 @@ __global__ __launch_bounds__(256) void k_sum_partials(const double* __restrict__
 #define SF21_SIZES(X) X(32) X(64) X(96) X(128) X(192) X(256) X(512)
 
 int balance_partials_count(long n3) {
+  // 2048 blocks: 8 waves per SIMD on 256 CUs, enough loads in flight
   long b = (n3 + 255) / 256;
-  return (int)(b < 1024 ? (b < 1 ? 1 : b) : 1024);
+  return (int)(b < 2048 ? (b < 1 ? 1 : b) : 2048);
 }
 
 void launch_balance(long n3, const float* a, const float* b, const float* c, CellConsts k,
